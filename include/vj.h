@@ -1,0 +1,230 @@
+/*
+ * vj.h -- C ABI of the MI355X (gfx950) hash-join and hashing library.
+ *
+ * This is the drop-in boundary for vgerber/vectorized-join's single-GPU hash
+ * join and hashing path. The reference has no ABI of its own (its executables
+ * #include the .cu files); each entry point below names the reference
+ * interface it replaces.
+ *
+ * Conventions
+ *  - Every function returns a vj_status; nothing calls exit() or throws across
+ *    the ABI (the reference's JoinStatus{successful,message},
+ *    src/join/join_provider.cu:11-35, becomes a status code plus
+ *    vj_last_error()).
+ *  - All table and output pointers are device pointers (HBM) unless stated.
+ *  - `stream` is a hipStream_t passed as void*; NULL is the default stream.
+ *  - A vj_ctx is not thread-safe; distinct contexts are independent.
+ *  - Tables hold fewer than 2^32 rows; row ids are uint32.
+ */
+#ifndef VJ_H
+#define VJ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VJ_ABI_VERSION 1
+#define VJ_MAX_KEY_COLS 4
+
+typedef enum vj_status {
+    VJ_OK = 0,
+    VJ_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad width, bad count */
+    VJ_ERR_UNKNOWN_HASH = 2,     /* unknown hash kind or name (hash.cu:515-541 silently did nothing) */
+    VJ_ERR_UNSUPPORTED = 3,      /* a valid request this build does not support */
+    VJ_ERR_BUDGET = 4,           /* admission refused: the HBM budget would be exceeded */
+    VJ_ERR_OUT_OF_MEMORY = 5,    /* hipMalloc failed */
+    VJ_ERR_CAPACITY = 6,         /* caller's output buffer too small; the count holds the size needed */
+    VJ_ERR_HIP = 7,              /* HIP runtime error */
+    VJ_ERR_NO_DEVICE = 8,        /* no usable gfx950 device */
+    VJ_ERR_NOT_FOUND = 9,        /* unknown handle / ticket */
+    VJ_ERR_INTERNAL = 10
+} vj_status;
+
+/* Hash algorithms, in the order of the reference's string dispatch
+ * (hash_func, src/hash/hash.cu:515-541; kernels hash.cu:84-505). */
+typedef enum vj_hash_kind {
+    VJ_HASH_FNV = 0,
+    VJ_HASH_CUSTOM_ADD = 1,
+    VJ_HASH_CUSTOM_ADD_SHIFT = 2,
+    VJ_HASH_CUSTOM_ADD_HW = 3,
+    VJ_HASH_CUSTOM_XOR = 4,
+    VJ_HASH_CUSTOM_XOR_SHIFT = 5,
+    VJ_HASH_CUSTOM_XOR_HW = 6,
+    VJ_HASH_CUSTOM_MULT = 7,
+    VJ_HASH_CUSTOM_MULT_SHIFT = 8,
+    VJ_HASH_CUSTOM_MULT_HW = 9,
+    VJ_HASH_CUSTOM_MULT_XOR_SHIFT = 10,
+    VJ_HASH_CUSTOM_N_MULT_XOR_SHIFT = 11,
+    VJ_HASH_CUSTOM_MULT_XOR_HW = 12,
+    VJ_HASH_KIND_COUNT = 13
+} vj_hash_kind;
+
+/* How the 32-bit hash word stream is formed from the key columns. */
+typedef enum vj_word_mode {
+    /* The reference layout: every key column is a 64-bit column_t
+     * (src/base/types.hpp:7) and contributes its low then high 32-bit word
+     * (src/hash/hash.cu:92-113 with chunk_t = uint2). 4-byte keys are widened
+     * (zero- or sign-extended). Bit-exact with the reference. */
+    VJ_WORDS_REFERENCE = 0,
+    /* Native: a 4-byte key column contributes one word, an 8-byte column two. */
+    VJ_WORDS_NATIVE = 1
+} vj_word_mode;
+
+/* A table of key columns in HBM, column-major (SoA). Replaces db_table
+ * (src/base/types.hpp:128-266), whose key columns are row-major uint64. */
+typedef struct vj_table {
+    uint64_t n_rows;
+    int32_t n_cols;      /* 1..VJ_MAX_KEY_COLS */
+    int32_t key_width;   /* 4 or 8 bytes per key column */
+    int32_t key_signed;  /* 4-byte keys: widen by sign extension (else zero extension) */
+    int32_t reserved;
+    const void *cols[VJ_MAX_KEY_COLS]; /* device pointers, n_rows values each */
+    /* Optional row ids carried with each row (the reference's primary_keys
+     * column, types.hpp:130). NULL: the 0-based row index is used. */
+    const uint32_t *rowids;
+} vj_table;
+
+typedef struct vj_hash_opts {
+    int32_t kind;       /* vj_hash_kind */
+    int32_t hash_bits;  /* 32 or 64 (reference HASH_BITS, src/base/config.h:28) */
+    int32_t chunk_bits; /* 8, 32, 64 or 128 (reference HASH_CHUNK_BITS, config.h:31) */
+    int32_t word_mode;  /* vj_word_mode */
+} vj_hash_opts;
+
+/* Defaults are the reference's compiled configuration: fnv, 64, 64, reference words. */
+void vj_hash_opts_default(vj_hash_opts *opts);
+
+const char *vj_status_str(vj_status status);
+
+/* Maps a reference algorithm name ("fnv", "custom_xor_hw", ...) to its kind.
+ * Replaces the string compare chain of hash_func (src/hash/hash.cu:515-541);
+ * an unknown name is VJ_ERR_UNKNOWN_HASH instead of a silent no-op. */
+vj_status vj_hash_kind_from_name(const char *name, int32_t *kind);
+const char *vj_hash_kind_name(int32_t kind);
+
+/*
+ * Hash every row of `table` into `out` (n_rows values of hash_bits/8 bytes).
+ * Replaces hash_func (src/hash/hash.cu:507-557) as used by
+ * JoinProvider::hash_table (src/join/join_provider.cuh:913-946): the word
+ * stream is all key columns from offset 0. Asynchronous on `stream`.
+ * Errors: VJ_ERR_UNKNOWN_HASH, VJ_ERR_INVALID_ARGUMENT (including a row width
+ * that is not a whole number of chunks, join_provider.cuh:926-928).
+ */
+vj_status vj_hash(const vj_table *table, const vj_hash_opts *opts, void *out, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Context (JoinProvider, src/join/join_provider.cuh:222-500)                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct vj_ctx vj_ctx;
+
+/* Creates a context on HIP device `device`. Replaces
+ * JoinProvider::JoinProvider(JoinConfig) (join_provider.cuh:225-228). */
+vj_status vj_ctx_create(int device, vj_ctx **ctx);
+/* Frees all device memory of the context (JoinProvider::~JoinProvider, :427-429). */
+vj_status vj_ctx_destroy(vj_ctx *ctx);
+/* HBM admission budget in bytes for scratch + outputs the context allocates
+ * (0 = unlimited). The reference's DeviceConfig::is_out_of_memory /
+ * MEMORY_TOLERANCE (join_provider.cuh:140-150, config.h:25). */
+vj_status vj_ctx_set_budget(vj_ctx *ctx, uint64_t bytes);
+/* Bytes of device memory the context currently holds. */
+vj_status vj_ctx_memory(vj_ctx *ctx, uint64_t *bytes);
+/* Detail message of the last failure on this context ("" if none). */
+const char *vj_last_error(const vj_ctx *ctx);
+
+typedef struct vj_join_opts {
+    vj_hash_opts hash;
+    int32_t passes;     /* radix passes: -1 = planner, 0, 1 or 2 */
+    int32_t bits1;      /* radix bits of pass 1 (0 = planner; 1..8) */
+    int32_t bits2;      /* radix bits of pass 2 (0 = planner; 1..8) */
+    int32_t one_based;  /* emit row ids + 1 (the reference's primary_keys = row + 1) */
+    uint64_t out_capacity_hint; /* expected pair count for vj_join (0 = probe rows) */
+} vj_join_opts;
+
+void vj_join_opts_default(vj_join_opts *opts);
+
+/* Join result: (build_rowid, probe_rowid) pairs in device memory owned by the
+ * caller after return; release with vj_pairs_free. */
+typedef struct vj_pairs {
+    uint64_t count;
+    uint32_t *build_rowids;
+    uint32_t *probe_rowids;
+} vj_pairs;
+
+/*
+ * Equi-join on all key columns: every (r, s) with build row r and probe row s
+ * whose key columns are all equal. Output order is unspecified (as in the
+ * reference, whose atomics make it nondeterministic); the pair set is exact.
+ * Replaces JoinStatus JoinProvider::join(db_table r, db_table s, db_table &rs)
+ * (join_provider.cuh:230-239, join_gpu :286-421). Like the reference it is
+ * synchronous; unlike it, inputs are never modified (the reference reuses
+ * them as swap space, join_provider.cuh:652-663).
+ * Output memory is allocated by the callee (as join_provider.cuh:975-976).
+ */
+vj_status vj_join(vj_ctx *ctx, const vj_table *build, const vj_table *probe, const vj_join_opts *opts,
+                  vj_pairs *out, void *stream);
+
+/*
+ * Same join into caller-provided device buffers of `capacity` pairs.
+ * *count receives the total number of matches. If it exceeds capacity the
+ * first `capacity` pairs (in unspecified order) are written and
+ * VJ_ERR_CAPACITY is returned. No allocation happens when the context's
+ * scratch is already large enough (the case for repeated joins of a shape).
+ */
+vj_status vj_join_into(vj_ctx *ctx, const vj_table *build, const vj_table *probe, const vj_join_opts *opts,
+                       uint32_t *out_build, uint32_t *out_probe, uint64_t capacity, uint64_t *count, void *stream);
+
+vj_status vj_pairs_free(vj_pairs *pairs);
+
+/* Per-join statistics of the last join on the context (JoinSummary,
+ * join_provider.cuh:54-106; times from hipEvents around each stage). */
+typedef struct vj_join_summary {
+    uint64_t build_rows, probe_rows, matches;
+    int32_t passes, bits1, bits2, retries;
+    uint64_t partitions, tasks;
+    double ms_total;     /* all stages, device time */
+    double ms_histogram, ms_pass1, ms_pass2, ms_build_probe;
+    uint64_t scratch_bytes;
+} vj_join_summary;
+
+vj_status vj_join_summary_get(vj_ctx *ctx, vj_join_summary *summary);
+/* Turn per-stage event timing on or off (off by default; the total is always timed). */
+vj_status vj_ctx_set_profiling(vj_ctx *ctx, int enabled);
+
+/*
+ * One radix-partition pass of a table on digit (hash >> shift) & (2^bits - 1):
+ * the rows are moved into `out` grouped by digit (digit 0 first), and
+ * hist[b] / offsets[b] receive the count and exclusive prefix per digit
+ * (uint32, device pointers, 2^bits entries each). Row order inside a digit
+ * is unspecified. Replaces partition_gpu (src/join/join_provider.cu:610-661);
+ * `out` must have n_rows capacity in every key column and in out->rowids.
+ */
+vj_status vj_partition(vj_ctx *ctx, const vj_table *table, const vj_hash_opts *hash, int32_t shift, int32_t bits,
+                       const vj_table *out, uint32_t *hist, uint32_t *offsets, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched joins (the reference runs bucket pairs as concurrent host threads */
+/* over streams, JoinProvider::probe, join_provider.cuh:792-911)            */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Runs n_joins independent joins as one pipeline: every stage is one launch
+ * over tiles of all joins, and build+probe is one persistent launch drawing
+ * (join, partition, chunk) tasks from a device work queue. All joins share
+ * opts (key schema and hash). Join i writes into out_build[i]/out_probe[i]
+ * (capacity caps[i]); counts[i] receives its match count. Returns
+ * VJ_ERR_CAPACITY if any join overflowed (its count tells the size needed).
+ * With use_graph != 0 the pipeline is captured once per shape in a hipGraph
+ * and replayed.
+ */
+vj_status vj_join_batch(vj_ctx *ctx, int32_t n_joins, const vj_table *builds, const vj_table *probes,
+                        const vj_join_opts *opts, uint32_t *const *out_build, uint32_t *const *out_probe,
+                        const uint64_t *caps, uint64_t *counts, int32_t use_graph, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VJ_H */

@@ -1,0 +1,50 @@
+// Host-side internals shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "vj.h"
+#include "vj_device.hpp"
+
+namespace vj {
+
+// Status with a message, converted to vj_status + vj_last_error at the ABI.
+struct Err {
+    vj_status code = VJ_OK;
+    std::string msg;
+    static Err ok() { return Err(); }
+    static Err make(vj_status c, std::string m) {
+        Err e;
+        e.code = c;
+        e.msg = std::move(m);
+        return e;
+    }
+    bool failed() const { return code != VJ_OK; }
+};
+
+#define VJ_HIP_TRY(expr)                                                                              \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            return ::vj::Err::make(VJ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+#define VJ_TRY(expr)                       \
+    do {                                   \
+        ::vj::Err e_ = (expr);             \
+        if (e_.failed()) return e_;        \
+    } while (0)
+
+// Validation shared by hashing and joining.
+Err check_table(const vj_table *t, bool need_rowids_absent_ok = true);
+Err make_hash_cfg(const vj_hash_opts *o, const vj_table *t, HashCfg *cfg);
+TView to_view(const vj_table *t);
+
+// vj_hash.hip
+Err launch_hash(const vj_table *t, const HashCfg &cfg, void *out, hipStream_t s, int num_cus);
+
+int device_cu_count(int device);
+
+}  // namespace vj

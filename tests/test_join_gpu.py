@@ -1,0 +1,252 @@
+"""GPU parity of the join pipeline (vj_join / vj_join_into / vj_partition /
+vj_join_batch) against the oracle: the sorted (build_rowid, probe_rowid) pair
+sets must be identical. Full-size runs are checked by size-independent
+properties (every output pair has equal keys; FK joins match each probe row
+exactly once)."""
+import numpy as np
+import pytest
+
+import oracle
+import vjoin
+import workloads
+from joinref import np_join, sorted_pairs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = vjoin.Context(0)
+    yield c
+    c.close()
+
+
+def _pairs(r, s):
+    return sorted_pairs(r.cpu().numpy().view(np.uint32), s.cpu().numpy().view(np.uint32))
+
+
+def _assert_same(got, want, what=""):
+    assert len(got[0]) == len(want[0]), (what, len(got[0]), len(want[0]))
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), what
+
+
+def _run(ctx, rcols, scols, rowids=(None, None), **kw):
+    dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+    rr = None if rowids[0] is None else workloads.to_device([rowids[0]])[0]
+    sr = None if rowids[1] is None else workloads.to_device([rowids[1]])[0]
+    key_signed = kw.pop("key_signed", False)
+    opts = vjoin.join_opts(**kw)
+    return ctx.join(vjoin.table(dr, rr, key_signed), vjoin.table(ds, sr, key_signed), opts)
+
+
+def test_plain_small_config_matches_oracle(ctx):
+    # build 2^16 u32 permutation, probe 2^20 uniform [0, 2^17), FNV-1a-32, 1 pass x 8 bits
+    rcols, scols = workloads.plain_small()
+    got = _pairs(*_run(ctx, rcols, scols, kind="fnv", hash_bits=32, passes=1, bits1=8))
+    want = sorted_pairs(*oracle.join(workloads.reference_rows(rcols), workloads.reference_rows(scols), "fnv", 32,
+                                     64, passes=1, bits1=8))
+    _assert_same(got, want)
+    s = ctx.summary()
+    assert s["passes"] == 1 and s["bits1"] == 8 and s["matches"] == len(want[0])
+
+
+@pytest.mark.parametrize("passes,b1,b2", [(0, 0, 0), (1, 4, 0), (1, 8, 0), (2, 3, 5), (2, 7, 7), (-1, 0, 0)])
+@pytest.mark.parametrize("kw", [4, 8])
+def test_passes_and_bits(ctx, passes, b1, b2, kw):
+    rng = np.random.default_rng((passes + 1) * 31 + b1 * 7 + b2 + kw)
+    nr, ns = 20000, 150000
+    dt = np.uint32 if kw == 4 else np.uint64
+    r = rng.integers(0, 60000, size=nr, dtype=np.uint64).astype(dt)
+    s = rng.integers(0, 60000, size=ns, dtype=np.uint64).astype(dt)
+    got = _pairs(*_run(ctx, [r], [s], passes=passes, bits1=b1, bits2=b2))
+    _assert_same(got, np_join(r.reshape(-1, 1).astype(np.uint64), s.reshape(-1, 1).astype(np.uint64)),
+                 (passes, b1, b2, kw))
+
+
+@pytest.mark.parametrize("kind,hash_bits", [("fnv", 32), ("custom_xor_hw", 64), ("custom_xor_hw", 32),
+                                            ("custom_add", 64), ("custom_mult_xor_hw", 32), ("fnv", 64)])
+def test_hash_kinds(ctx, kind, hash_bits):
+    rng = np.random.default_rng(hash_bits)
+    r = rng.integers(0, 2**40, size=30000, dtype=np.uint64)
+    s = np.concatenate([r[rng.integers(0, len(r), 60000)], rng.integers(0, 2**40, 60000, dtype=np.uint64)])
+    got = _pairs(*_run(ctx, [r], [s], kind=kind, hash_bits=hash_bits, passes=2, bits1=4, bits2=4))
+    want = sorted_pairs(*oracle.join(r.reshape(-1, 1), s.reshape(-1, 1), kind, hash_bits, 64, passes=2, bits1=4,
+                                     bits2=4))
+    _assert_same(got, want, kind)
+
+
+@pytest.mark.parametrize("kw,ncols", [(4, 2), (4, 3), (4, 4), (8, 2), (8, 3), (8, 4)])
+def test_multicolumn_keys_with_duplicates(ctx, kw, ncols):
+    rng = np.random.default_rng(kw * 10 + ncols)
+    dt = np.uint32 if kw == 4 else np.uint64
+    distinct = rng.integers(0, 7, size=(3000, ncols), dtype=np.uint64).astype(dt)
+    r = np.repeat(distinct, 3, axis=0)[rng.permutation(9000)]
+    s = distinct[rng.integers(0, len(distinct), 40000)]
+    s[::5, 0] += 11  # some misses
+    got = _pairs(*_run(ctx, [r[:, c].copy() for c in range(ncols)], [s[:, c].copy() for c in range(ncols)],
+                       passes=2, bits1=3, bits2=4))
+    want = np_join(r.astype(np.uint64), s.astype(np.uint64))
+    _assert_same(got, want, (kw, ncols))
+
+
+def test_explicit_rowids_and_one_based(ctx):
+    rng = np.random.default_rng(3)
+    r = rng.integers(0, 5000, size=8000, dtype=np.uint64).astype(np.uint32)
+    s = rng.integers(0, 5000, size=30000, dtype=np.uint64).astype(np.uint32)
+    rr = rng.permutation(10**6)[:8000].astype(np.uint32)
+    sr = rng.permutation(10**6)[:30000].astype(np.uint32)
+    got = _pairs(*_run(ctx, [r], [s], rowids=(rr, sr), passes=1, bits1=5, one_based=True))
+    want = np_join(r.reshape(-1, 1).astype(np.uint64), s.reshape(-1, 1).astype(np.uint64), rr + 1, sr + 1)
+    _assert_same(got, want)
+
+
+def test_signed_keys(ctx):
+    rng = np.random.default_rng(4)
+    r = rng.integers(-3000, 3000, size=5000).astype(np.int32)
+    s = rng.integers(-3000, 3000, size=20000).astype(np.int32)
+    got = _pairs(*_run(ctx, [r], [s], key_signed=True, passes=1, bits1=6))
+    want = sorted_pairs(*oracle.join(oracle.reference_rows([r.view(np.uint32)], signed=True),
+                                     oracle.reference_rows([s.view(np.uint32)], signed=True), passes=1, bits1=6))
+    _assert_same(got, want)
+
+
+def test_heavy_duplicates_chunk_the_build_side(ctx):
+    # one key repeated far beyond an LDS build chunk: R is cut into chunks, each probed with the whole S side
+    r = np.full(10000, 42, dtype=np.uint32)
+    r[::3] = 7
+    s = np.array([42, 7, 1, 42] * 300, dtype=np.uint32)
+    got = _pairs(*_run(ctx, [r], [s], passes=1, bits1=2))
+    want = np_join(r.reshape(-1, 1).astype(np.uint64), s.reshape(-1, 1).astype(np.uint64))
+    assert len(want[0]) > 3_000_000
+    _assert_same(got, want)
+
+
+def test_zipf_skewed_multicolumn_sample(ctx):
+    rcols, scols = workloads.multicol_skew(log_distinct=14, log_s=19)
+    got = _pairs(*_run(ctx, rcols, scols, kind="custom_xor_hw", passes=2, bits1=4, bits2=4))
+    want = np_join(workloads.reference_rows(rcols), workloads.reference_rows(scols))
+    _assert_same(got, want)
+
+
+@pytest.mark.parametrize("nr,ns", [(0, 100), (100, 0), (0, 0), (1, 1), (5, 70000)])
+def test_empty_and_tiny(ctx, nr, ns):
+    import torch
+    rng = np.random.default_rng(nr + ns)
+    r = rng.integers(0, 10, size=nr, dtype=np.uint64).astype(np.uint32)
+    s = rng.integers(0, 10, size=ns, dtype=np.uint64).astype(np.uint32)
+    dr = torch.from_numpy(r.view(np.int32)).cuda() if nr else torch.empty(0, dtype=torch.int32, device="cuda")
+    ds = torch.from_numpy(s.view(np.int32)).cuda() if ns else torch.empty(0, dtype=torch.int32, device="cuda")
+    for passes in (0, 1, 2):
+        got = _pairs(*ctx.join(vjoin.table([dr]), vjoin.table([ds]), vjoin.join_opts(passes=passes, bits1=2,
+                                                                                      bits2=2)))
+        want = np_join(r.reshape(-1, 1).astype(np.uint64), s.reshape(-1, 1).astype(np.uint64))
+        _assert_same(got, want, (nr, ns, passes))
+
+
+def test_join_into_capacity(ctx):
+    import torch
+    rcols, scols = workloads.plain_small(log_r=12, log_s=16)
+    dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+    opts = vjoin.join_opts(passes=1, bits1=6)
+    want = np_join(workloads.reference_rows(rcols), workloads.reference_rows(scols))
+    m = len(want[0])
+    small_r = torch.empty(100, dtype=torch.int32, device="cuda")
+    small_s = torch.empty(100, dtype=torch.int32, device="cuda")
+    rc, cnt = ctx.join_into(vjoin.table(dr), vjoin.table(ds), opts, small_r, small_s)
+    assert vjoin.STATUS[rc] == "VJ_ERR_CAPACITY" and cnt == m
+    out_r = torch.empty(m, dtype=torch.int32, device="cuda")
+    out_s = torch.empty(m, dtype=torch.int32, device="cuda")
+    rc, cnt = ctx.join_into(vjoin.table(dr), vjoin.table(ds), opts, out_r, out_s)
+    assert rc == 0 and cnt == m
+    _assert_same(_pairs(out_r, out_s), want)
+
+
+def test_vj_join_grows_output_beyond_hint(ctx):
+    # 1:n join whose output is 4x the probe side: vj_join reruns build+probe with an exact buffer
+    rng = np.random.default_rng(9)
+    base = rng.permutation(4000).astype(np.uint32)
+    r = np.repeat(base, 4)
+    s = base[rng.integers(0, 4000, 50000)]
+    got = _run(ctx, [r], [s], passes=1, bits1=4)
+    assert ctx.summary()["retries"] == 1
+    want = np_join(r.reshape(-1, 1).astype(np.uint64), s.reshape(-1, 1).astype(np.uint64))
+    _assert_same(_pairs(*got), want)
+
+
+@pytest.mark.parametrize("bits,shift", [(8, 0), (5, 3), (1, 17)])
+def test_partition_matches_oracle(ctx, bits, shift):
+    import torch
+    rng = np.random.default_rng(bits)
+    k = rng.integers(0, 2**63, size=100000, dtype=np.uint64)
+    dk = workloads.to_device([k])
+    out_k = torch.empty_like(dk[0])
+    out_id = torch.empty(len(k), dtype=torch.int32, device="cuda")
+    hist = torch.empty(1 << bits, dtype=torch.int32, device="cuda")
+    off = torch.empty(1 << bits, dtype=torch.int32, device="cuda")
+    hopts = vjoin.hash_opts("fnv", 64, 64)
+    rc = ctx.partition(vjoin.table(dk), hopts, shift, bits, vjoin.table([out_k], out_id), hist, off)
+    assert rc == 0, ctx.last_error()
+    torch.cuda.synchronize()
+    h = oracle.hash_rows(k.reshape(-1, 1), "fnv", 64, 64)
+    whist, woff, wperm = oracle.radix_partition(h, shift, bits)
+    assert np.array_equal(hist.cpu().numpy().astype(np.uint64), whist)
+    assert np.array_equal(off.cpu().numpy().astype(np.uint64), woff)
+    ids = out_id.cpu().numpy().view(np.uint32)
+    keys = out_k.cpu().numpy().view(np.uint64)
+    assert np.array_equal(keys, k[ids])  # rows moved with their ids
+    for b in range(1 << bits):  # same multiset of rows per digit
+        seg = slice(int(woff[b]), int(woff[b] + whist[b]))
+        assert np.array_equal(np.sort(ids[seg]), np.sort(wperm[seg]))
+
+
+def test_batch_of_uneven_joins(ctx):
+    import torch
+    joins = workloads.batched(n_joins=24, log_min=8, log_max=15)
+    builds, probes, outs_r, outs_s, caps, keep = [], [], [], [], [], []
+    for rcols, scols in joins:
+        dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+        keep += [dr, ds]
+        builds.append(vjoin.table(dr))
+        probes.append(vjoin.table(ds))
+        cap = len(scols[0])
+        outs_r.append(torch.empty(cap, dtype=torch.int32, device="cuda"))
+        outs_s.append(torch.empty(cap, dtype=torch.int32, device="cuda"))
+        caps.append(cap)
+    rc, counts = ctx.join_batch(builds, probes, vjoin.join_opts(), outs_r, outs_s, caps)
+    assert rc == 0, ctx.last_error()
+    for i, (rcols, scols) in enumerate(joins):
+        want = np_join(workloads.reference_rows(rcols), workloads.reference_rows(scols))
+        got = _pairs(outs_r[i][:counts[i]], outs_s[i][:counts[i]])
+        _assert_same(got, want, i)
+
+
+def test_budget_admission(ctx):
+    rcols, scols = workloads.plain_small(log_r=14, log_s=18)
+    dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+    c2 = vjoin.Context(0)
+    c2.set_budget(1 << 16)
+    with pytest.raises(vjoin.VJError) as e:
+        c2.join(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts(passes=1, bits1=8))
+    assert e.value.name == "VJ_ERR_BUDGET"
+    c2.set_budget(0)
+    r, s = c2.join(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts(passes=1, bits1=8))
+    assert len(r) > 0
+    c2.close()
+
+
+def test_large_fk_join_full_size(ctx):
+    """2^24 unique int64 build keys, 2^27 FK probe keys, 2 passes: every probe row
+    matches exactly one build row with an equal key."""
+    import torch
+    rcols, scols = workloads.large_fk()
+    dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+    n = len(scols[0])
+    out_r = torch.empty(n, dtype=torch.int32, device="cuda")
+    out_s = torch.empty(n, dtype=torch.int32, device="cuda")
+    rc, cnt = ctx.join_into(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts(passes=2), out_r, out_s)
+    assert rc == 0, ctx.last_error()
+    assert cnt == n
+    torch.cuda.synchronize()
+    sid = out_s.long()
+    assert torch.equal(torch.sort(sid).values, torch.arange(n, device="cuda"))
+    assert torch.equal(dr[0][out_r.long()], ds[0][sid])

@@ -1,0 +1,729 @@
+// vj_ctx: device arenas, the join planner, pipeline launch, and the join /
+// partition / batch entry points of the C ABI.
+//
+// Reference counterparts: JoinProvider (src/join/join_provider.cuh:222-500)
+// with join_gpu's stage order (:286-421), DeviceConfig's memory admission
+// (:124-220), partition_gpu (src/join/join_provider.cu:610-661) and the
+// bucket-pair probing loop (join_provider.cuh:792-911), here a single
+// device-side work queue instead of host threads.
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "vj_internal.hpp"
+#include "vj_pipeline.hpp"
+
+using namespace vj;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+constexpr uint32_t kDefaultCapS = 16384;  // probe rows per build+probe task
+
+inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+enum Ev { EV_START, EV_HIST, EV_SCAN, EV_P1, EV_P2, EV_PROBE, EV_COUNT };
+
+}  // namespace
+
+struct vj_ctx {
+    int device = 0;
+    int cus = 256;
+    uint64_t budget = 0;
+    bool profiling = false;
+    std::string last_error;
+    vj_join_summary summary{};
+    DevBuf tuples, hist, base, p2pre, ptask, status, meta;
+    bool hist_dirty = true;
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+    hipEvent_t ev[EV_COUNT] = {};
+};
+
+namespace {
+
+// --- memory ----------------------------------------------------------------
+
+uint64_t held_bytes(const vj_ctx *c) {
+    return c->tuples.bytes + c->hist.bytes + c->base.bytes + c->p2pre.bytes + c->ptask.bytes + c->status.bytes +
+           c->meta.bytes;
+}
+
+void free_buf(DevBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+void free_all(vj_ctx *c) {
+    free_buf(c->tuples);
+    free_buf(c->hist);
+    free_buf(c->base);
+    free_buf(c->p2pre);
+    free_buf(c->ptask);
+    free_buf(c->status);
+    free_buf(c->meta);
+    c->hist_dirty = true;
+}
+
+Err ensure(DevBuf &b, size_t bytes) {
+    if (b.bytes >= bytes) return Err::ok();
+    free_buf(b);
+    size_t want = align_up(bytes + bytes / 8);
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+        (void)hipGetLastError();
+        b.p = nullptr;
+        return Err::make(VJ_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(want) + " bytes failed");
+    }
+    b.bytes = want;
+    return Err::ok();
+}
+
+Err ensure_pinned(vj_ctx *c, size_t bytes) {
+    if (c->pinned_bytes >= bytes) return Err::ok();
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    c->pinned = nullptr;
+    c->pinned_bytes = 0;
+    size_t want = align_up(bytes * 2);
+    VJ_HIP_TRY(hipHostMalloc(&c->pinned, want, hipHostMallocDefault));
+    c->pinned_bytes = want;
+    return Err::ok();
+}
+
+// --- planning ----------------------------------------------------------------
+
+struct JoinSpec {
+    const vj_table *r, *s;
+    uint32_t *out_r, *out_s;
+    uint64_t cap;
+};
+
+struct Plan {
+    std::vector<JobDesc> jobs;
+    std::vector<JoinDesc> joins;
+    std::vector<uint32_t> hist_block0, p1_tile0;
+    std::vector<size_t> fin_off, mid_off;  // byte offsets in the tuples arena (SIZE_MAX: none)
+    size_t tuples_bytes = 0;
+    uint64_t hist_words = 0, base_words = 0, p2pre_words = 0, ptask_words = 0, status_words = 0;
+    uint32_t max_hist_bins = 1, total_hist_blocks = 0, total_p1_tiles = 0, total_p2_tiles_max = 0;
+    bool any_part = false, any_p2 = false;
+    LaunchShape shape{};
+    HashCfg hash{};
+    int kb = 0;
+    // meta layout
+    size_t off_jobs = 0, off_joins = 0, off_hb0 = 0, off_pt0 = 0, upload_bytes = 0;
+    size_t off_jp2 = 0, off_p2t0 = 0, off_jtasks = 0, off_task0 = 0, off_count = 0, off_ctl = 0, meta_bytes = 0;
+};
+
+int ceil_log2(uint64_t v) {
+    int b = 0;
+    while ((1ull << b) < v) b++;
+    return b;
+}
+
+// Radix passes and bits for one join (the reference fixes one radix width
+// per level, join_provider.cuh:296-300, and recurses until a partition fits,
+// :569-571; here the partition target is half an LDS build chunk).
+Err plan_bits(uint64_t nr, int kb, const vj_join_opts *o, int *P, int *b1, int *b2) {
+    const uint64_t target = build_cap(kb) / 2;
+    const int need = nr > target ? ceil_log2((nr + target - 1) / target) : 0;
+    int passes = o->passes;
+    if (passes < -1 || passes > 2) return Err::make(VJ_ERR_INVALID_ARGUMENT, "passes must be -1, 0, 1 or 2");
+    if (passes == -1) passes = need == 0 ? 0 : (need <= kMaxPassBits ? 1 : 2);
+    if (o->bits1 < 0 || o->bits2 < 0 || o->bits1 > kMaxPassBits || o->bits2 > kMaxPassBits)
+        return Err::make(VJ_ERR_UNSUPPORTED, "radix bits per pass must be 1..8");
+    *P = passes;
+    if (passes == 0) {
+        *b1 = *b2 = 0;
+    } else if (passes == 1) {
+        *b1 = o->bits1 ? o->bits1 : std::max(1, std::min(need, kMaxPassBits));
+        *b2 = 0;
+    } else {
+        int total = std::max(2, std::min(need, kMaxHistBits));
+        int x1 = o->bits1 ? o->bits1 : std::min(kMaxPassBits, (total + 1) / 2);
+        int x2 = o->bits2 ? o->bits2 : std::max(1, std::min(kMaxPassBits, total - x1));
+        *b1 = x1;
+        *b2 = x2;
+    }
+    if (*b1 + *b2 > kMaxHistBits) return Err::make(VJ_ERR_UNSUPPORTED, "bits1 + bits2 must be <= 14");
+    return Err::ok();
+}
+
+Err check_pair(const vj_table *r, const vj_table *s) {
+    VJ_TRY(check_table(r));
+    VJ_TRY(check_table(s));
+    // join_gpu asserts equal column counts (join_provider.cuh:291)
+    if (r->n_cols != s->n_cols || r->key_width != s->key_width || (r->key_signed != 0) != (s->key_signed != 0))
+        return Err::make(VJ_ERR_INVALID_ARGUMENT, "build and probe tables must have the same key columns");
+    return Err::ok();
+}
+
+void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0) {
+    JobDesc jd;
+    memset(&jd, 0, sizeof(jd));
+    jd.in = to_view(t);
+    jd.n = (uint32_t)t->n_rows;
+    jd.b1 = P >= 1 ? b1 : 0;
+    jd.b2 = P >= 2 ? b2 : 0;
+    jd.shift0 = shift0;
+    const uint32_t T = pass_tile(pl.kb);
+    const uint32_t nb = 1u << (jd.b1 + jd.b2);
+    jd.base_off = (uint32_t)pl.base_words;
+    pl.base_words += (jd.b1 ? nb : 1) + 1;
+    size_t row_bytes_cols = 0;
+    if (jd.b1) {
+        pl.any_part = true;
+        jd.hist_off = (uint32_t)pl.hist_words;
+        pl.hist_words += nb;
+        pl.max_hist_bins = std::max(pl.max_hist_bins, nb);
+        jd.hist_blocks = jd.n ? (uint32_t)std::min<uint64_t>(512, std::max<uint64_t>(1, ((uint64_t)jd.n + 65535) / 65536)) : 0;
+        jd.p1_tiles = (uint32_t)(((uint64_t)jd.n + T - 1) / T);
+        jd.p1_stat_off = pl.status_words;
+        pl.status_words += (uint64_t)jd.p1_tiles << jd.b1;
+        if (jd.b2) {
+            pl.any_p2 = true;
+            jd.p2pre_off = (uint32_t)pl.p2pre_words;
+            pl.p2pre_words += (1u << jd.b1) + 1;
+            jd.p2_tiles_max = jd.n ? (uint32_t)(((uint64_t)jd.n + T - 1) / T) + (1u << jd.b1) : 0;
+            jd.p2_stat_off = pl.status_words;
+            pl.status_words += (uint64_t)jd.p2_tiles_max << jd.b2;
+        }
+        row_bytes_cols = (size_t)t->key_width * jd.n;
+    }
+    pl.hist_block0.push_back(pl.total_hist_blocks);
+    pl.total_hist_blocks += jd.hist_blocks;
+    pl.p1_tile0.push_back(pl.total_p1_tiles);
+    pl.total_p1_tiles += jd.p1_tiles;
+    pl.total_p2_tiles_max += jd.p2_tiles_max;
+    // tuple buffers: fin (and mid for two passes), SoA key columns + row ids
+    size_t fin = SIZE_MAX, mid = SIZE_MAX;
+    if (jd.b1) {
+        const size_t one = align_up(row_bytes_cols) * t->n_cols + align_up((size_t)4 * jd.n);
+        fin = pl.tuples_bytes;
+        pl.tuples_bytes += one;
+        if (jd.b2) {
+            mid = pl.tuples_bytes;
+            pl.tuples_bytes += one;
+        }
+    }
+    pl.fin_off.push_back(fin);
+    pl.mid_off.push_back(mid);
+    pl.jobs.push_back(jd);
+}
+
+void finish_layout(Plan &pl) {
+    const size_t nj = pl.jobs.size(), nn = pl.joins.size();
+    size_t o = 0;
+    pl.off_jobs = o;
+    o = align_up(o + nj * sizeof(JobDesc));
+    pl.off_joins = o;
+    o = align_up(o + nn * sizeof(JoinDesc));
+    pl.off_hb0 = o;
+    o = align_up(o + (nj + 1) * 4);
+    pl.off_pt0 = o;
+    o = align_up(o + (nj + 1) * 4);
+    pl.upload_bytes = o;
+    pl.off_jp2 = o;
+    o = align_up(o + nj * 4);
+    pl.off_p2t0 = o;
+    o = align_up(o + (nj + 1) * 4);
+    pl.off_jtasks = o;
+    o = align_up(o + nn * 8);
+    pl.off_task0 = o;
+    o = align_up(o + (nn + 1) * 8);
+    pl.off_count = o;
+    o = align_up(o + nn * 8);
+    pl.off_ctl = o;
+    o = align_up(o + sizeof(Ctl));
+    pl.meta_bytes = o;
+    pl.hist_block0.push_back(pl.total_hist_blocks);
+    pl.p1_tile0.push_back(pl.total_p1_tiles);
+}
+
+Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_opts *o) {
+    const vj_table *t0 = specs[0].r;
+    VJ_TRY(make_hash_cfg(&o->hash, t0, &pl.hash));
+    pl.kb = t0->key_width * t0->n_cols;
+    pl.shape.hp = hash_path(pl.hash);
+    pl.shape.kw = t0->key_width;
+    pl.shape.nc = t0->n_cols;
+    for (size_t i = 0; i < specs.size(); i++) {
+        const JoinSpec &js = specs[i];
+        VJ_TRY(check_pair(js.r, js.s));
+        if (js.r->n_cols != t0->n_cols || js.r->key_width != t0->key_width ||
+            (js.r->key_signed != 0) != (t0->key_signed != 0))
+            return Err::make(VJ_ERR_INVALID_ARGUMENT, "all joins of a batch must share the key columns' shape");
+        int P, b1, b2;
+        VJ_TRY(plan_bits(js.r->n_rows, pl.kb, o, &P, &b1, &b2));
+        JoinDesc jd;
+        memset(&jd, 0, sizeof(jd));
+        jd.job_r = (uint32_t)pl.jobs.size();
+        add_job(pl, js.r, P, b1, b2, 0);
+        jd.job_s = (uint32_t)pl.jobs.size();
+        add_job(pl, js.s, P, b1, b2, 0);
+        jd.out_r = js.out_r;
+        jd.out_s = js.out_s;
+        jd.cap = js.cap;
+        jd.cap_r = build_cap(pl.kb);
+        jd.cap_s = kDefaultCapS;
+        jd.nparts = 1u << (P >= 1 ? b1 : 0) << (P >= 2 ? b2 : 0);
+        jd.ptask_off = (uint32_t)pl.ptask_words;
+        pl.ptask_words += jd.nparts + 1;
+        jd.slot_shift = (P >= 1 ? b1 : 0) + (P >= 2 ? b2 : 0);
+        jd.one_based = o->one_based ? 1 : 0;
+        pl.joins.push_back(jd);
+    }
+    finish_layout(pl);
+    return Err::ok();
+}
+
+uint64_t plan_scratch_bytes(const Plan &pl) {
+    return align_up(pl.tuples_bytes) + align_up(pl.hist_words * 4) + align_up(pl.base_words * 4) +
+           align_up(pl.p2pre_words * 4) + align_up(pl.ptask_words * 8) + align_up(pl.status_words * 8) +
+           align_up(pl.meta_bytes);
+}
+
+// Admission (DeviceConfig::is_out_of_memory, join_provider.cuh:148-150): the
+// scratch of this pipeline plus `extra` output bytes must fit the budget.
+Err admit(vj_ctx *c, const Plan &pl, uint64_t extra) {
+    if (!c->budget) return Err::ok();
+    const uint64_t need = plan_scratch_bytes(pl) + extra;
+    if (need > c->budget)
+        return Err::make(VJ_ERR_BUDGET, "Not enough memory for swap tables and hashes: need " + std::to_string(need) +
+                                            " bytes, budget " + std::to_string(c->budget));
+    // drop larger-than-needed arenas when keeping them would break the budget
+    uint64_t keep = 0;
+    keep += std::max<uint64_t>(c->tuples.bytes, pl.tuples_bytes) + std::max<uint64_t>(c->hist.bytes, pl.hist_words * 4) +
+            std::max<uint64_t>(c->base.bytes, pl.base_words * 4) + std::max<uint64_t>(c->p2pre.bytes, pl.p2pre_words * 4) +
+            std::max<uint64_t>(c->ptask.bytes, pl.ptask_words * 8) +
+            std::max<uint64_t>(c->status.bytes, pl.status_words * 8) + std::max<uint64_t>(c->meta.bytes, pl.meta_bytes);
+    if (keep + extra > c->budget) free_all(c);
+    return Err::ok();
+}
+
+Err allocate(vj_ctx *c, Plan &pl) {
+    VJ_TRY(ensure(c->tuples, std::max<size_t>(pl.tuples_bytes, kAlign)));
+    const bool hist_grew = c->hist.bytes < pl.hist_words * 4;
+    VJ_TRY(ensure(c->hist, std::max<size_t>(pl.hist_words * 4, kAlign)));
+    if (hist_grew || c->hist_dirty) {
+        VJ_HIP_TRY(hipMemset(c->hist.p, 0, c->hist.bytes));
+        c->hist_dirty = false;
+    }
+    VJ_TRY(ensure(c->base, std::max<size_t>(pl.base_words * 4, kAlign)));
+    VJ_TRY(ensure(c->p2pre, std::max<size_t>(pl.p2pre_words * 4, kAlign)));
+    VJ_TRY(ensure(c->ptask, std::max<size_t>(pl.ptask_words * 8, kAlign)));
+    VJ_TRY(ensure(c->status, std::max<size_t>(pl.status_words * 8, kAlign)));
+    VJ_TRY(ensure(c->meta, pl.meta_bytes));
+    VJ_TRY(ensure_pinned(c, pl.meta_bytes));
+    // tuple buffer pointers
+    char *tb = static_cast<char *>(c->tuples.p);
+    for (size_t j = 0; j < pl.jobs.size(); j++) {
+        JobDesc &jd = pl.jobs[j];
+        const size_t col = align_up((size_t)pl.shape.kw * jd.n);
+        if (pl.fin_off[j] != SIZE_MAX) {
+            char *p = tb + pl.fin_off[j];
+            for (int cc = 0; cc < 4; cc++) jd.fin_cols[cc] = cc < pl.shape.nc ? p + cc * col : nullptr;
+            jd.fin_ids = reinterpret_cast<uint32_t *>(p + pl.shape.nc * col);
+        }
+        if (pl.mid_off[j] != SIZE_MAX) {
+            char *p = tb + pl.mid_off[j];
+            for (int cc = 0; cc < 4; cc++) jd.mid_cols[cc] = cc < pl.shape.nc ? p + cc * col : nullptr;
+            jd.mid_ids = reinterpret_cast<uint32_t *>(p + pl.shape.nc * col);
+        }
+    }
+    return Err::ok();
+}
+
+PipeArgs make_args(vj_ctx *c, const Plan &pl) {
+    char *m = static_cast<char *>(c->meta.p);
+    PipeArgs a;
+    memset(&a, 0, sizeof(a));
+    a.jobs = reinterpret_cast<const JobDesc *>(m + pl.off_jobs);
+    a.joins = reinterpret_cast<const JoinDesc *>(m + pl.off_joins);
+    a.hist_block0 = reinterpret_cast<const uint32_t *>(m + pl.off_hb0);
+    a.p1_tile0 = reinterpret_cast<const uint32_t *>(m + pl.off_pt0);
+    a.job_p2_tiles = reinterpret_cast<uint32_t *>(m + pl.off_jp2);
+    a.p2_tile0 = reinterpret_cast<uint32_t *>(m + pl.off_p2t0);
+    a.join_tasks = reinterpret_cast<unsigned long long *>(m + pl.off_jtasks);
+    a.task0 = reinterpret_cast<unsigned long long *>(m + pl.off_task0);
+    a.out_count = reinterpret_cast<unsigned long long *>(m + pl.off_count);
+    a.ctl = reinterpret_cast<Ctl *>(m + pl.off_ctl);
+    a.hist = static_cast<uint32_t *>(c->hist.p);
+    a.base = static_cast<uint32_t *>(c->base.p);
+    a.p2pre = static_cast<uint32_t *>(c->p2pre.p);
+    a.ptask = static_cast<unsigned long long *>(c->ptask.p);
+    a.status = static_cast<unsigned long long *>(c->status.p);
+    a.status_words = pl.status_words;
+    a.n_jobs = (uint32_t)pl.jobs.size();
+    a.n_joins = (uint32_t)pl.joins.size();
+    a.total_hist_blocks = pl.total_hist_blocks;
+    a.total_p1_tiles = pl.total_p1_tiles;
+    a.total_p2_tiles_max = pl.total_p2_tiles_max;
+    a.max_hist_bins = pl.max_hist_bins;
+    a.pass_tile = pass_tile(pl.kb);
+    a.hash = pl.hash;
+    return a;
+}
+
+Err upload(vj_ctx *c, const Plan &pl, hipStream_t s) {
+    char *h = static_cast<char *>(c->pinned);
+    memcpy(h + pl.off_jobs, pl.jobs.data(), pl.jobs.size() * sizeof(JobDesc));
+    memcpy(h + pl.off_joins, pl.joins.data(), pl.joins.size() * sizeof(JoinDesc));
+    memcpy(h + pl.off_hb0, pl.hist_block0.data(), pl.hist_block0.size() * 4);
+    memcpy(h + pl.off_pt0, pl.p1_tile0.data(), pl.p1_tile0.size() * 4);
+    VJ_HIP_TRY(hipMemcpyAsync(c->meta.p, h, pl.upload_bytes, hipMemcpyHostToDevice, s));
+    return Err::ok();
+}
+
+#define VJ_LAUNCH(expr)                                                                         \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            c->hist_dirty = true;                                                               \
+            return Err::make(VJ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+        }                                                                                       \
+    } while (0)
+
+// Launches every stage on `s` (no host synchronisation).
+Err launch_stages(vj_ctx *c, const Plan &pl, const PipeArgs &a, hipStream_t s, bool with_probe) {
+    const bool prof = c->profiling;
+    VJ_HIP_TRY(hipEventRecord(c->ev[EV_START], s));
+    VJ_LAUNCH(launch_prepare(a, s));
+    if (pl.any_part) VJ_LAUNCH(launch_hist(a, pl.shape, s));
+    if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_HIST], s));
+    VJ_LAUNCH(launch_scan(a, s));
+    if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_SCAN], s));
+    if (pl.any_part) VJ_LAUNCH(launch_pass(a, pl.shape, 1, s));
+    if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_P1], s));
+    if (pl.any_p2) VJ_LAUNCH(launch_pass(a, pl.shape, 2, s));
+    if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_P2], s));
+    if (with_probe && !pl.joins.empty())
+        VJ_LAUNCH(launch_probe(a, pl.shape, c->cus * probe_blocks_per_cu(pl.shape), s));
+    VJ_HIP_TRY(hipEventRecord(c->ev[EV_PROBE], s));
+    return Err::ok();
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.f;
+    }
+    return ms;
+}
+
+void fill_summary(vj_ctx *c, const Plan &pl, const JoinSpec &js, uint64_t matches, int retries) {
+    vj_join_summary &s = c->summary;
+    memset(&s, 0, sizeof(s));
+    s.build_rows = js.r->n_rows;
+    s.probe_rows = js.s->n_rows;
+    s.matches = matches;
+    const JobDesc &jr = pl.jobs[0];
+    s.passes = jr.b2 ? 2 : (jr.b1 ? 1 : 0);
+    s.bits1 = jr.b1;
+    s.bits2 = jr.b2;
+    s.retries = retries;
+    s.partitions = pl.joins[0].nparts;
+    s.ms_total = elapsed(c->ev[EV_START], c->ev[EV_PROBE]);
+    if (c->profiling) {
+        s.ms_histogram = elapsed(c->ev[EV_START], c->ev[EV_HIST]);
+        s.ms_pass1 = elapsed(c->ev[EV_SCAN], c->ev[EV_P1]);
+        s.ms_pass2 = elapsed(c->ev[EV_P1], c->ev[EV_P2]);
+        s.ms_build_probe = elapsed(c->ev[EV_P2], c->ev[EV_PROBE]);
+    }
+    s.scratch_bytes = held_bytes(c);
+}
+
+vj_status fail(vj_ctx *c, const Err &e) {
+    if (c) c->last_error = e.msg;
+    return e.code;
+}
+
+Err set_device(vj_ctx *c) {
+    VJ_HIP_TRY(hipSetDevice(c->device));
+    return Err::ok();
+}
+
+// Runs the pipeline for `specs`; counts[i] receives join i's match count.
+// When `grow` is set, joins whose output overflowed get a fresh buffer of the
+// exact size (hipMalloc, owned by the caller) and only build+probe is rerun.
+Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hipStream_t s,
+              std::vector<uint64_t> &counts, bool grow, uint64_t extra_bytes, int *retries_out) {
+    Plan pl;
+    VJ_TRY(build_join_plan(pl, specs, o));
+    VJ_TRY(admit(c, pl, extra_bytes));
+    VJ_TRY(allocate(c, pl));
+    VJ_TRY(upload(c, pl, s));
+    PipeArgs a = make_args(c, pl);
+    VJ_TRY(launch_stages(c, pl, a, s, true));
+    char *h = static_cast<char *>(c->pinned);
+    unsigned long long *hc = reinterpret_cast<unsigned long long *>(h + pl.off_count);
+    VJ_HIP_TRY(hipMemcpyAsync(hc, static_cast<char *>(c->meta.p) + pl.off_count, pl.joins.size() * 8,
+                              hipMemcpyDeviceToHost, s));
+    VJ_HIP_TRY(hipStreamSynchronize(s));
+    counts.assign(hc, hc + pl.joins.size());
+    int retries = 0;
+    if (grow) {
+        bool again = false;
+        for (size_t i = 0; i < specs.size(); i++) {
+            if (counts[i] > specs[i].cap) {
+                if (specs[i].out_r) (void)hipFree(specs[i].out_r);
+                if (specs[i].out_s) (void)hipFree(specs[i].out_s);
+                specs[i].out_r = specs[i].out_s = nullptr;
+                if (hipMalloc(&specs[i].out_r, counts[i] * 4) != hipSuccess ||
+                    hipMalloc(&specs[i].out_s, counts[i] * 4) != hipSuccess) {
+                    (void)hipGetLastError();
+                    return Err::make(VJ_ERR_OUT_OF_MEMORY, "output allocation failed");
+                }
+                specs[i].cap = counts[i];
+                pl.joins[i].out_r = specs[i].out_r;
+                pl.joins[i].out_s = specs[i].out_s;
+                pl.joins[i].cap = specs[i].cap;
+                again = true;
+            }
+        }
+        if (again) {
+            retries = 1;
+            memcpy(h + pl.off_joins, pl.joins.data(), pl.joins.size() * sizeof(JoinDesc));
+            VJ_HIP_TRY(hipMemcpyAsync(static_cast<char *>(c->meta.p) + pl.off_joins, h + pl.off_joins,
+                                      pl.joins.size() * sizeof(JoinDesc), hipMemcpyHostToDevice, s));
+            VJ_LAUNCH(launch_reset_probe(a, s));
+            VJ_LAUNCH(launch_probe(a, pl.shape, c->cus * probe_blocks_per_cu(pl.shape), s));
+            VJ_HIP_TRY(hipEventRecord(c->ev[EV_PROBE], s));
+            VJ_HIP_TRY(hipMemcpyAsync(hc, static_cast<char *>(c->meta.p) + pl.off_count, pl.joins.size() * 8,
+                                      hipMemcpyDeviceToHost, s));
+            VJ_HIP_TRY(hipStreamSynchronize(s));
+            counts.assign(hc, hc + pl.joins.size());
+        }
+    }
+    if (retries_out) *retries_out = retries;
+    if (specs.size() == 1) fill_summary(c, pl, specs[0], counts[0], retries);
+    return Err::ok();
+}
+
+}  // namespace
+
+extern "C" {
+
+vj_status vj_ctx_create(int device, vj_ctx **out) {
+    if (!out) return VJ_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return VJ_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= n) return VJ_ERR_INVALID_ARGUMENT;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return VJ_ERR_NO_DEVICE;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return VJ_ERR_NO_DEVICE;  // built for gfx950 only
+    if (hipSetDevice(device) != hipSuccess) return VJ_ERR_HIP;
+    vj_ctx *c = new vj_ctx();
+    c->device = device;
+    c->cus = device_cu_count(device);
+    for (int i = 0; i < EV_COUNT; i++) {
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+            for (int k = 0; k < i; k++) (void)hipEventDestroy(c->ev[k]);
+            delete c;
+            return VJ_ERR_HIP;
+        }
+    }
+    *out = c;
+    return VJ_OK;
+}
+
+vj_status vj_ctx_destroy(vj_ctx *c) {
+    if (!c) return VJ_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    free_all(c);
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    for (int i = 0; i < EV_COUNT; i++)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    delete c;
+    return VJ_OK;
+}
+
+vj_status vj_ctx_set_budget(vj_ctx *c, uint64_t bytes) {
+    if (!c) return VJ_ERR_INVALID_ARGUMENT;
+    c->budget = bytes;
+    return VJ_OK;
+}
+
+vj_status vj_ctx_memory(vj_ctx *c, uint64_t *bytes) {
+    if (!c || !bytes) return VJ_ERR_INVALID_ARGUMENT;
+    *bytes = held_bytes(c);
+    return VJ_OK;
+}
+
+vj_status vj_ctx_set_profiling(vj_ctx *c, int enabled) {
+    if (!c) return VJ_ERR_INVALID_ARGUMENT;
+    c->profiling = enabled != 0;
+    return VJ_OK;
+}
+
+const char *vj_last_error(const vj_ctx *c) { return c ? c->last_error.c_str() : ""; }
+
+void vj_join_opts_default(vj_join_opts *o) {
+    if (!o) return;
+    vj_hash_opts_default(&o->hash);
+    o->passes = -1;
+    o->bits1 = 0;
+    o->bits2 = 0;
+    o->one_based = 0;
+    o->out_capacity_hint = 0;
+}
+
+vj_status vj_join_into(vj_ctx *c, const vj_table *build, const vj_table *probe, const vj_join_opts *opts,
+                       uint32_t *out_build, uint32_t *out_probe, uint64_t capacity, uint64_t *count, void *stream) {
+    if (!c || !count) return VJ_ERR_INVALID_ARGUMENT;
+    c->last_error.clear();
+    *count = 0;
+    vj_join_opts d;
+    vj_join_opts_default(&d);
+    if (!opts) opts = &d;
+    if (capacity && (!out_build || !out_probe)) return fail(c, Err::make(VJ_ERR_INVALID_ARGUMENT, "output is NULL"));
+    Err e = check_pair(build, probe);
+    if (e.failed()) return fail(c, e);
+    e = set_device(c);
+    if (e.failed()) return fail(c, e);
+    std::vector<JoinSpec> specs{{build, probe, out_build, out_probe, capacity}};
+    std::vector<uint64_t> counts;
+    e = run_joins(c, specs, opts, (hipStream_t)stream, counts, false, 0, nullptr);
+    if (e.failed()) return fail(c, e);
+    *count = counts[0];
+    if (counts[0] > capacity)
+        return fail(c, Err::make(VJ_ERR_CAPACITY, "output holds " + std::to_string(capacity) + " pairs, join has " +
+                                                      std::to_string(counts[0])));
+    return VJ_OK;
+}
+
+vj_status vj_join(vj_ctx *c, const vj_table *build, const vj_table *probe, const vj_join_opts *opts, vj_pairs *out,
+                  void *stream) {
+    if (!c || !out) return VJ_ERR_INVALID_ARGUMENT;
+    c->last_error.clear();
+    memset(out, 0, sizeof(*out));
+    vj_join_opts d;
+    vj_join_opts_default(&d);
+    if (!opts) opts = &d;
+    Err e = check_pair(build, probe);
+    if (e.failed()) return fail(c, e);
+    e = set_device(c);
+    if (e.failed()) return fail(c, e);
+    uint64_t cap = opts->out_capacity_hint ? opts->out_capacity_hint : std::max<uint64_t>(probe->n_rows, 1);
+    std::vector<JoinSpec> specs{{build, probe, nullptr, nullptr, cap}};
+    if (hipMalloc(&specs[0].out_r, cap * 4) != hipSuccess || hipMalloc(&specs[0].out_s, cap * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        if (specs[0].out_r) (void)hipFree(specs[0].out_r);
+        return fail(c, Err::make(VJ_ERR_OUT_OF_MEMORY, "output allocation failed"));
+    }
+    std::vector<uint64_t> counts;
+    e = run_joins(c, specs, opts, (hipStream_t)stream, counts, true, cap * 8, nullptr);
+    if (e.failed()) {
+        if (specs[0].out_r) (void)hipFree(specs[0].out_r);
+        if (specs[0].out_s) (void)hipFree(specs[0].out_s);
+        return fail(c, e);
+    }
+    out->count = counts[0];
+    out->build_rowids = specs[0].out_r;
+    out->probe_rowids = specs[0].out_s;
+    return VJ_OK;
+}
+
+vj_status vj_pairs_free(vj_pairs *p) {
+    if (!p) return VJ_ERR_INVALID_ARGUMENT;
+    if (p->build_rowids) (void)hipFree(p->build_rowids);
+    if (p->probe_rowids) (void)hipFree(p->probe_rowids);
+    p->build_rowids = p->probe_rowids = nullptr;
+    p->count = 0;
+    return VJ_OK;
+}
+
+vj_status vj_join_summary_get(vj_ctx *c, vj_join_summary *s) {
+    if (!c || !s) return VJ_ERR_INVALID_ARGUMENT;
+    *s = c->summary;
+    return VJ_OK;
+}
+
+vj_status vj_partition(vj_ctx *c, const vj_table *table, const vj_hash_opts *hash, int32_t shift, int32_t bits,
+                       const vj_table *out, uint32_t *hist, uint32_t *offsets, void *stream) {
+    if (!c) return VJ_ERR_INVALID_ARGUMENT;
+    c->last_error.clear();
+    Err e = check_table(table);
+    if (e.failed()) return fail(c, e);
+    e = check_table(out);
+    if (e.failed()) return fail(c, e);
+    if (bits < 1 || bits > kMaxPassBits) return fail(c, Err::make(VJ_ERR_UNSUPPORTED, "bits must be 1..8"));
+    if (shift < 0 || shift > 63) return fail(c, Err::make(VJ_ERR_INVALID_ARGUMENT, "shift must be 0..63"));
+    if (out->n_cols != table->n_cols || out->key_width != table->key_width || out->n_rows < table->n_rows ||
+        !out->rowids || !hist || !offsets)
+        return fail(c, Err::make(VJ_ERR_INVALID_ARGUMENT, "output table must match the input and carry row ids"));
+    e = set_device(c);
+    if (e.failed()) return fail(c, e);
+    hipStream_t s = (hipStream_t)stream;
+    Plan pl;
+    e = make_hash_cfg(hash, table, &pl.hash);
+    if (e.failed()) return fail(c, e);
+    pl.kb = table->key_width * table->n_cols;
+    pl.shape.hp = hash_path(pl.hash);
+    pl.shape.kw = table->key_width;
+    pl.shape.nc = table->n_cols;
+    add_job(pl, table, 1, bits, 0, (uint32_t)shift);
+    finish_layout(pl);
+    pl.tuples_bytes = 0;  // writes into the caller's table
+    pl.fin_off[0] = SIZE_MAX;
+    e = admit(c, pl, 0);
+    if (!e.failed()) e = allocate(c, pl);
+    if (e.failed()) return fail(c, e);
+    JobDesc &jd = pl.jobs[0];
+    for (int cc = 0; cc < 4; cc++) jd.fin_cols[cc] = cc < out->n_cols ? const_cast<void *>(out->cols[cc]) : nullptr;
+    jd.fin_ids = const_cast<uint32_t *>(out->rowids);
+    e = upload(c, pl, s);
+    if (e.failed()) return fail(c, e);
+    PipeArgs a = make_args(c, pl);
+    e = launch_stages(c, pl, a, s, false);
+    if (e.failed()) return fail(c, e);
+    hipError_t he = launch_emit_hist(a.base + jd.base_off, 1u << bits, hist, offsets, s);
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    if (he != hipSuccess) return fail(c, Err::make(VJ_ERR_HIP, hipGetErrorString(he)));
+    return VJ_OK;
+}
+
+vj_status vj_join_batch(vj_ctx *c, int32_t n_joins, const vj_table *builds, const vj_table *probes,
+                        const vj_join_opts *opts, uint32_t *const *out_build, uint32_t *const *out_probe,
+                        const uint64_t *caps, uint64_t *counts, int32_t use_graph, void *stream) {
+    if (!c || n_joins <= 0 || !builds || !probes || !out_build || !out_probe || !caps || !counts)
+        return VJ_ERR_INVALID_ARGUMENT;
+    (void)use_graph;
+    c->last_error.clear();
+    vj_join_opts d;
+    vj_join_opts_default(&d);
+    if (!opts) opts = &d;
+    Err e = set_device(c);
+    if (e.failed()) return fail(c, e);
+    std::vector<JoinSpec> specs((size_t)n_joins);
+    for (int i = 0; i < n_joins; i++) {
+        specs[i] = {&builds[i], &probes[i], out_build[i], out_probe[i], caps[i]};
+        if (caps[i] && (!out_build[i] || !out_probe[i]))
+            return fail(c, Err::make(VJ_ERR_INVALID_ARGUMENT, "output is NULL"));
+    }
+    std::vector<uint64_t> cnt;
+    e = run_joins(c, specs, opts, (hipStream_t)stream, cnt, false, 0, nullptr);
+    if (e.failed()) return fail(c, e);
+    bool over = false;
+    for (int i = 0; i < n_joins; i++) {
+        counts[i] = cnt[i];
+        over |= cnt[i] > caps[i];
+    }
+    if (over) return fail(c, Err::make(VJ_ERR_CAPACITY, "an output buffer of the batch is too small"));
+    return VJ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,751 @@
+// The join pipeline's kernels (gfx950). One pipeline runs any number of joins;
+// every kernel draws its tiles from all of them.
+//
+//   k_prepare   reset control words and per-join counters               (1 block)
+//   k_hist      2D radix histogram (pass-1 x pass-2 digit) per job: LDS
+//               counters, one global atomic per non-empty bin per block;
+//               also clears this pipeline's look-back status words
+//   k_scan      per join: exclusive scans of both histograms -> partition
+//               bases; pass-2 tile counts; build/probe task prefix
+//   k_scan2     prefixes across jobs / joins (pass-2 tiles, tasks)      (1 block)
+//   k_pass      one radix pass: tile -> per-wave LDS digit ranks -> decoupled
+//               look-back per digit over the earlier tiles of the same
+//               sequence -> LDS staging ordered by digit -> contiguous runs
+//               per digit written to HBM
+//   k_probe     persistent; (join, partition, R-chunk, S-chunk) tasks from a
+//               device queue: open-addressing table of the R chunk in LDS,
+//               probe with the S chunk, 64-lane ballot compaction of matches
+//               into per-wave LDS buffers, one global atomic per buffer flush
+//
+// The reference's counterparts are histogram_kernel / prefix_sum_kernel /
+// element_swap_kernel (src/join/join_provider.cu:315-380) and the build /
+// probe / extract kernels (:382-608). Digits are LSB-first on the hash as in
+// the reference (:303-305, :326, :362); the table slot uses the hash bits
+// above the consumed digits, like the reference's key_offset (:394-395, :409);
+// a pair matches iff every key column is equal (:422-437).
+#include "vj_pipeline.hpp"
+
+namespace vj {
+
+namespace {
+
+__device__ __forceinline__ unsigned long long ld_status(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_status(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Largest i in [0, n) with a[i] <= x, for non-decreasing a with a[0] <= x.
+template <typename T> __device__ __forceinline__ uint32_t upper_index(const T *a, uint32_t n, T x) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Same search done by a whole wave: 64 probes per round, so an array of
+// 2^16 entries needs three dependent loads instead of sixteen.
+template <typename T> __device__ __forceinline__ uint32_t wave_upper_index(const T *a, uint32_t n, T x) {
+    const uint32_t lane = lane_id();
+    uint32_t lo = 0, len = n;
+    while (len > 1) {
+        const uint32_t step = (len + 63) >> 6;
+        const uint32_t i = lo + lane * step;
+        const bool ok = lane == 0 || (i < lo + len && a[i] <= x);
+        const uint64_t m = __ballot(ok);
+        const uint32_t last = 63 - __builtin_clzll(m);
+        const uint32_t end = lo + len;
+        lo = lo + last * step;
+        len = (end - lo) < step ? (end - lo) : step;
+    }
+    return lo;
+}
+
+template <typename T> __device__ __forceinline__ T wave_incl_scan(T v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T o = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v += o;
+    }
+    return v;
+}
+
+// Exclusive scan over the block; `sh` holds >= blockDim/64 + 1 values.
+template <typename T> __device__ T block_excl_scan(T v, T *sh, T *total) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    T inc = wave_incl_scan(v);
+    if (lane == 63) sh[w] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T acc = 0;
+        for (uint32_t i = 0; i < nw; i++) {
+            T t = sh[i];
+            sh[i] = acc;
+            acc += t;
+        }
+        sh[nw] = acc;
+    }
+    __syncthreads();
+    T r = sh[w] + inc - v;
+    *total = sh[nw];
+    __syncthreads();
+    return r;
+}
+
+// Exclusive scan of in[0..n) into out[0..n], total into out[n]; in may equal out.
+template <typename T> __device__ void block_scan_array(const T *in, T *out, uint32_t n, T *sh) {
+    const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+    const uint32_t lo = threadIdx.x * per < n ? threadIdx.x * per : n;
+    const uint32_t hi = lo + per < n ? lo + per : n;
+    T sum = 0;
+    for (uint32_t i = lo; i < hi; i++) sum += in[i];
+    T total;
+    T pre = block_excl_scan<T>(sum, sh, &total);
+    for (uint32_t i = lo; i < hi; i++) {
+        T v = in[i];
+        out[i] = pre;
+        pre += v;
+    }
+    if (threadIdx.x == 0) out[n] = total;
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+
+__global__ void k_prepare(PipeArgs a) {
+    if (threadIdx.x == 0) {
+        a.ctl->p1_ticket = 0;
+        a.ctl->p2_ticket = 0;
+        a.ctl->p2_total = 0;
+        a.ctl->task_ticket = 0;
+        a.ctl->task_total = 0;
+    }
+    for (uint32_t j = threadIdx.x; j < a.n_joins; j += blockDim.x) {
+        a.out_count[j] = 0;
+        a.join_tasks[j] = 0;
+    }
+    for (uint32_t j = threadIdx.x; j < a.n_jobs; j += blockDim.x) a.job_p2_tiles[j] = 0;
+}
+
+constexpr int kHistThreads = 1024;
+
+template <int HP, int KW, int NC>
+__global__ __launch_bounds__(kHistThreads) void k_hist(PipeArgs a) {
+    extern __shared__ uint32_t lds_hist[];
+    const uint32_t b = blockIdx.x;
+    {
+        const uint64_t per = (a.status_words + gridDim.x - 1) / gridDim.x;
+        const uint64_t lo = (uint64_t)b * per;
+        const uint64_t hi = lo + per < a.status_words ? lo + per : a.status_words;
+        for (uint64_t i = lo + threadIdx.x; i < hi; i += kHistThreads) a.status[i] = 0;
+    }
+    const uint32_t j = upper_index(a.hist_block0, a.n_jobs, b);
+    const JobDesc &job = a.jobs[j];
+    const uint32_t nb = 1u << (job.b1 + job.b2);
+    for (uint32_t i = threadIdx.x; i < nb; i += kHistThreads) lds_hist[i] = 0;
+    __syncthreads();
+    const uint32_t k = b - a.hist_block0[j];
+    const uint64_t lo = (uint64_t)job.n * k / job.hist_blocks;
+    const uint64_t hi = (uint64_t)job.n * (k + 1) / job.hist_blocks;
+    const uint32_t m1 = (1u << job.b1) - 1, m2 = (1u << job.b2) - 1;
+    const uint32_t s1 = job.shift0, s2 = job.shift0 + job.b1;
+    const HashCfg hc = a.hash;
+    constexpr int U = 4;
+    uint64_t i = lo + threadIdx.x;
+    for (; i + (U - 1) * kHistThreads < hi; i += U * kHistThreads) {
+        Key<KW, NC> key[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) key[u] = load_key<KW, NC>(job.in, i + u * kHistThreads);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            uint64_t h = hash_key<HP, KW, NC>(key[u], hc);
+            uint32_t q = (((uint32_t)(h >> s1) & m1) << job.b2) | ((uint32_t)(h >> s2) & m2);
+            atomicAdd(&lds_hist[q], 1u);
+        }
+    }
+    for (; i < hi; i += kHistThreads) {
+        Key<KW, NC> key = load_key<KW, NC>(job.in, i);
+        uint64_t h = hash_key<HP, KW, NC>(key, hc);
+        uint32_t q = (((uint32_t)(h >> s1) & m1) << job.b2) | ((uint32_t)(h >> s2) & m2);
+        atomicAdd(&lds_hist[q], 1u);
+    }
+    __syncthreads();
+    uint32_t *g = a.hist + job.hist_off;
+    for (uint32_t t = threadIdx.x; t < nb; t += kHistThreads) {
+        uint32_t c = lds_hist[t];
+        if (c) atomicAdd(&g[t], c);
+    }
+}
+
+// One block of 1024 threads per job: partition bases, pass-2 tile prefix.
+__global__ __launch_bounds__(1024) void k_scan(PipeArgs a) {
+    __shared__ uint32_t sh32[20];
+    const uint32_t jj = blockIdx.x;
+    const JobDesc &job = a.jobs[jj];
+    uint32_t *base = a.base + job.base_off;
+    if (job.b1 == 0) {
+        if (threadIdx.x == 0) {
+            base[0] = 0;
+            base[1] = job.n;
+        }
+        return;
+    }
+    const uint32_t nb = 1u << (job.b1 + job.b2);
+    uint32_t *hist = a.hist + job.hist_off;
+    block_scan_array<uint32_t>(hist, base, nb, sh32);
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) hist[i] = 0;  // ready for the next pipeline
+    if (job.b2 > 0) {
+        const uint32_t n1 = 1u << job.b1;
+        uint32_t *pre = a.p2pre + job.p2pre_off;
+        for (uint32_t d1 = threadIdx.x; d1 < n1; d1 += blockDim.x) {
+            const uint32_t cnt = base[(d1 + 1) << job.b2] - base[d1 << job.b2];
+            pre[d1] = (cnt + a.pass_tile - 1) / a.pass_tile;
+        }
+        __syncthreads();
+        block_scan_array<uint32_t>(pre, pre, n1, sh32);
+        if (threadIdx.x == 0) a.job_p2_tiles[jj] = pre[n1];
+    }
+}
+
+// One block of 1024 threads per join: build/probe tasks = R chunks x S chunks
+// of every partition pair, as an exclusive prefix over partitions.
+__global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
+    __shared__ unsigned long long sh64[20];
+    const uint32_t jn = blockIdx.x;
+    const JoinDesc &jd = a.joins[jn];
+    const uint32_t *rb = a.base + a.jobs[jd.job_r].base_off;
+    const uint32_t *sb = a.base + a.jobs[jd.job_s].base_off;
+    unsigned long long *pt = a.ptask + jd.ptask_off;
+    for (uint32_t q = threadIdx.x; q < jd.nparts; q += blockDim.x) {
+        const uint32_t nr = rb[q + 1] - rb[q], ns = sb[q + 1] - sb[q];
+        unsigned long long t = 0;
+        if (nr && ns) t = (unsigned long long)((nr + jd.cap_r - 1) / jd.cap_r) * ((ns + jd.cap_s - 1) / jd.cap_s);
+        pt[q] = t;
+    }
+    __syncthreads();
+    block_scan_array<unsigned long long>(pt, pt, jd.nparts, sh64);
+    if (threadIdx.x == 0) a.join_tasks[jn] = pt[jd.nparts];
+}
+
+// Resets the queue and the output counters to run build+probe again (after
+// an output buffer was found too small).
+__global__ void k_reset_probe(PipeArgs a) {
+    if (threadIdx.x == 0) a.ctl->task_ticket = 0;
+    for (uint32_t j = threadIdx.x; j < a.n_joins; j += blockDim.x) a.out_count[j] = 0;
+}
+
+__global__ __launch_bounds__(1024) void k_scan2(PipeArgs a) {
+    __shared__ uint32_t sh32[20];
+    __shared__ unsigned long long sh64[20];
+    block_scan_array<uint32_t>(a.job_p2_tiles, a.p2_tile0, a.n_jobs, sh32);
+    block_scan_array<unsigned long long>(a.join_tasks, a.task0, a.n_joins, sh64);
+    if (threadIdx.x == 0) {
+        a.ctl->p2_total = a.p2_tile0[a.n_jobs];
+        a.ctl->task_total = a.task0[a.n_joins];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Radix pass
+// ---------------------------------------------------------------------------
+
+struct PassTile {
+    TView src;
+    void *dst_cols[4];
+    uint32_t *dst_ids;
+    const uint32_t *dbase;  // digit d's output base is dbase[d * dstride]
+    unsigned long long *seq;  // status of tile 0 of this tile's sequence
+    uint32_t dstride;
+    uint32_t lo, hi;        // rows of src
+    uint32_t bins, shift;
+    uint32_t local;         // tile index inside its sequence
+    uint32_t valid;
+};
+
+template <int HP, int KW, int NC>
+__global__ __launch_bounds__(kThreads) void k_pass(PipeArgs a, int pass) {
+    using KT = typename KeyT<KW>::T;
+    constexpr int KB = KW * NC;
+    constexpr int ITEMS = pass_items(KB);
+    constexpr int T = kThreads * ITEMS;
+    extern __shared__ __align__(16) unsigned char smem[];
+    KT *skeys = reinterpret_cast<KT *>(smem);                         // [NC][T]
+    uint32_t *sids = reinterpret_cast<uint32_t *>(skeys + NC * T);     // [T]
+    uint32_t *wcnt = sids + T;                                         // [kWaves][256]
+    uint32_t *loc = wcnt + kWaves * 256;                               // [256]
+    uint32_t *gdst = loc + 256;                                        // [256]
+    uint8_t *sdig = reinterpret_cast<uint8_t *>(gdst + 256);           // [T]
+    __shared__ PassTile ti;
+    __shared__ uint32_t sh_scan[8];
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (uint32_t i = tid; i < kWaves * 256; i += kThreads) wcnt[i] = 0;
+    if (w == 0) {
+        uint32_t ticket = 0;
+        if (lane == 0) ticket = atomicAdd(pass == 1 ? &a.ctl->p1_ticket : &a.ctl->p2_ticket, 1u);
+        ticket = __shfl(ticket, 0, 64);
+        uint32_t valid, j, local;
+        if (pass == 1) {
+            valid = ticket < a.total_p1_tiles;
+            j = valid ? wave_upper_index(a.p1_tile0, a.n_jobs, ticket) : 0;
+            local = valid ? ticket - a.p1_tile0[j] : 0;
+        } else {
+            valid = ticket < a.ctl->p2_total;
+            j = valid ? wave_upper_index(a.p2_tile0, a.n_jobs, ticket) : 0;
+            local = valid ? ticket - a.p2_tile0[j] : 0;
+        }
+        uint32_t d1 = 0, lt = local;
+        const JobDesc &job = a.jobs[j];
+        const uint32_t *pre = a.p2pre + job.p2pre_off;
+        if (valid && pass == 2) {
+            d1 = wave_upper_index(pre, 1u << job.b1, local);
+            lt = local - pre[d1];
+        }
+        if (lane == 0) {
+            ti.valid = valid;
+            if (valid) {
+                const uint32_t *base = a.base + job.base_off;
+                if (pass == 1) {
+                    ti.src = job.in;
+                    for (int c = 0; c < 4; c++) ti.dst_cols[c] = job.b2 ? job.mid_cols[c] : job.fin_cols[c];
+                    ti.dst_ids = job.b2 ? job.mid_ids : job.fin_ids;
+                    ti.dbase = base;
+                    ti.dstride = 1u << job.b2;
+                    ti.lo = local * T;
+                    ti.hi = ti.lo + T < job.n ? ti.lo + T : job.n;
+                    ti.bins = 1u << job.b1;
+                    ti.shift = job.shift0;
+                    ti.seq = a.status + job.p1_stat_off;
+                    ti.local = local;
+                } else {
+                    for (int c = 0; c < 4; c++) ti.src.cols[c] = job.mid_cols[c];
+                    ti.src.rowids = job.mid_ids;
+                    ti.src.n = job.n;
+                    for (int c = 0; c < 4; c++) ti.dst_cols[c] = job.fin_cols[c];
+                    ti.dst_ids = job.fin_ids;
+                    ti.dbase = base + (d1 << job.b2);
+                    ti.dstride = 1;
+                    const uint32_t p_lo = base[d1 << job.b2], p_hi = base[(d1 + 1) << job.b2];
+                    ti.lo = p_lo + lt * T;
+                    ti.hi = ti.lo + T < p_hi ? ti.lo + T : p_hi;
+                    ti.bins = 1u << job.b2;
+                    ti.shift = job.shift0 + job.b1;
+                    ti.seq = a.status + job.p2_stat_off + (uint64_t)pre[d1] * (1u << job.b2);
+                    ti.local = lt;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (!ti.valid) return;
+
+    const uint32_t lo = ti.lo, hi = ti.hi, bins = ti.bins, shift = ti.shift, mask = bins - 1;
+    const TView src = ti.src;
+    const HashCfg hc = a.hash;
+
+    // 1. load ITEMS rows per lane (wave w owns a contiguous 64*ITEMS segment), rank by digit
+    Key<KW, NC> keys[ITEMS];
+    uint32_t ids[ITEMS];
+    uint32_t pk[ITEMS];  // digit | rank << 8, or ~0 for no row
+    const uint32_t wbase = lo + w * 64 * ITEMS + lane;
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        const uint32_t i = wbase + it * 64;
+        if (i < hi) {
+            keys[it] = load_key<KW, NC>(src, i);
+            ids[it] = src.rowids ? src.rowids[i] : i;
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        const uint32_t i = wbase + it * 64;
+        pk[it] = ~0u;
+        if (i < hi) {
+            uint64_t h = hash_key<HP, KW, NC>(keys[it], hc);
+            uint32_t d = (uint32_t)(h >> shift) & mask;
+            uint32_t r = atomicAdd(&wcnt[w * 256 + d], 1u);
+            pk[it] = d | (r << 8);
+        }
+    }
+    __syncthreads();
+
+    // 2. per digit: wave prefixes, tile count, local offsets, look-back
+    uint32_t cnt = 0;
+    if (tid < bins) {
+#pragma unroll
+        for (int ww = 0; ww < kWaves; ww++) {
+            uint32_t c = wcnt[ww * 256 + tid];
+            wcnt[ww * 256 + tid] = cnt;
+            cnt += c;
+        }
+    }
+    uint32_t total;
+    const uint32_t my_loc = block_excl_scan<uint32_t>(cnt, sh_scan, &total);
+    if (tid < bins) {
+        loc[tid] = my_loc;
+        unsigned long long *my = ti.seq + (uint64_t)ti.local * bins + tid;
+        uint32_t excl = 0;
+        if (ti.local == 0) {
+            st_status(my, kFlagIncl | cnt);
+        } else {
+            st_status(my, kFlagAgg | cnt);
+            const unsigned long long *p = my - bins;
+            while (true) {
+                unsigned long long v = ld_status(p);
+                while ((v >> 62) == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    v = ld_status(p);
+                }
+                excl += (uint32_t)v;
+                if ((v >> 62) == 2) break;
+                p -= bins;
+            }
+            st_status(my, kFlagIncl | (unsigned long long)(excl + cnt));
+        }
+        gdst[tid] = ti.dbase[tid * ti.dstride] + excl - my_loc;
+    }
+    __syncthreads();
+
+    // 3. stage rows in LDS ordered by digit
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        if (pk[it] != ~0u) {
+            const uint32_t d = pk[it] & 255u;
+            const uint32_t slot = loc[d] + wcnt[w * 256 + d] + (pk[it] >> 8);
+#pragma unroll
+            for (int c = 0; c < NC; c++) skeys[c * T + slot] = keys[it].c[c];
+            sids[slot] = ids[it];
+            sdig[slot] = (uint8_t)d;
+        }
+    }
+    __syncthreads();
+
+    // 4. write runs: consecutive slots of one digit go to consecutive rows
+    const uint32_t n = hi - lo;
+    for (uint32_t k = tid; k < n; k += kThreads) {
+        const uint32_t dst = gdst[sdig[k]] + k;
+#pragma unroll
+        for (int c = 0; c < NC; c++) static_cast<KT *>(ti.dst_cols[c])[dst] = skeys[c * T + k];
+        ti.dst_ids[dst] = sids[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Build + probe
+// ---------------------------------------------------------------------------
+
+struct ProbeTask {
+    TView r, s;
+    uint32_t *out_r, *out_s;
+    unsigned long long *count;
+    uint64_t cap;
+    uint32_t r_lo, r_n, s_lo, s_n;
+    uint32_t nslots, shift, one_based, tag_shift;
+    uint32_t dup;
+    uint32_t done;
+};
+
+constexpr int kProbeUnroll = 8;
+
+template <int KW, int NC>
+__device__ __forceinline__ void flush_wave(uint32_t *obr, uint32_t *obs, uint32_t &wn, const ProbeTask &t) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t lane = lane_id();
+    unsigned long long pos = 0;
+    if (lane == 0) pos = atomicAdd(t.count, (unsigned long long)wn);
+    pos = __shfl(pos, 0, 64);
+    for (uint32_t k = lane; k < wn; k += 64) {
+        const unsigned long long o = pos + k;
+        if (o < t.cap) {
+            t.out_r[o] = obr[k];
+            t.out_s[o] = obs[k];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    wn = 0;
+}
+
+template <int HP, int KW, int NC>
+__global__ __launch_bounds__(kThreads) void k_probe(PipeArgs a) {
+    using KT = typename KeyT<KW>::T;
+    constexpr int KB = KW * NC;
+    constexpr int CAP = build_cap(KB);
+    extern __shared__ __align__(16) unsigned char smem[];
+    KT *rk = reinterpret_cast<KT *>(smem);                       // [NC][CAP]
+    uint32_t *rid = reinterpret_cast<uint32_t *>(rk + NC * CAP);  // [CAP]
+    uint32_t *slots = rid + CAP;                                  // [2 * CAP]
+    uint32_t *ob = slots + 2 * CAP;                               // [kWaves][2][kOutBuf]
+    __shared__ ProbeTask t;
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t *obr = ob + w * 2 * kOutBuf, *obs = obr + kOutBuf;
+    const HashCfg hc = a.hash;
+    const unsigned long long total = a.ctl->task_total;
+    const uint32_t tag_shift = hc.bits - 16;
+
+    while (true) {
+        if (w == 0) {
+            unsigned long long task = 0;
+            if (lane == 0) task = atomicAdd(&a.ctl->task_ticket, 1ull);
+            task = __shfl(task, 0, 64);
+            if (task >= total) {
+                if (lane == 0) t.done = 1;
+            } else {
+                const uint32_t jn = wave_upper_index(a.task0, a.n_joins, task);
+                const JoinDesc &jd = a.joins[jn];
+                const unsigned long long *pt = a.ptask + jd.ptask_off;
+                const unsigned long long local = task - a.task0[jn];
+                const uint32_t q = wave_upper_index(pt, jd.nparts, local);
+                if (lane == 0) {
+                    const JobDesc &jr = a.jobs[jd.job_r];
+                    const JobDesc &js = a.jobs[jd.job_s];
+                    const uint32_t *rb = a.base + jr.base_off, *sb = a.base + js.base_off;
+                    const uint32_t nr = rb[q + 1] - rb[q], ns = sb[q + 1] - sb[q];
+                    const uint32_t s_chunks = (ns + jd.cap_s - 1) / jd.cap_s;
+                    const uint32_t k = (uint32_t)(local - pt[q]);
+                    const uint32_t rc = k / s_chunks, sc = k - rc * s_chunks;
+                    if (jr.b1) {
+                        for (int c = 0; c < 4; c++) t.r.cols[c] = jr.fin_cols[c];
+                        t.r.rowids = jr.fin_ids;
+                    } else {
+                        t.r = jr.in;
+                    }
+                    if (js.b1) {
+                        for (int c = 0; c < 4; c++) t.s.cols[c] = js.fin_cols[c];
+                        t.s.rowids = js.fin_ids;
+                    } else {
+                        t.s = js.in;
+                    }
+                    t.r_lo = rb[q] + rc * jd.cap_r;
+                    t.r_n = nr - rc * jd.cap_r < jd.cap_r ? nr - rc * jd.cap_r : jd.cap_r;
+                    t.s_lo = sb[q] + sc * jd.cap_s;
+                    t.s_n = ns - sc * jd.cap_s < jd.cap_s ? ns - sc * jd.cap_s : jd.cap_s;
+                    uint32_t ns2 = 64;
+                    while (ns2 < 2 * t.r_n) ns2 <<= 1;
+                    t.nslots = ns2;
+                    t.shift = jd.slot_shift;
+                    t.one_based = jd.one_based;
+                    t.out_r = jd.out_r;
+                    t.out_s = jd.out_s;
+                    t.cap = jd.cap;
+                    t.count = &a.out_count[jn];
+                    t.dup = 0;
+                    t.done = 0;
+                }
+            }
+        }
+        __syncthreads();
+        if (t.done) break;
+
+        // build: load the R chunk, then insert (index + 1, 16-bit tag) with linear probing
+        const uint32_t r_lo = t.r_lo, r_n = t.r_n, nslots = t.nslots, smask = nslots - 1, shift = t.shift;
+        for (uint32_t i = tid; i < nslots; i += kThreads) slots[i] = 0;
+        for (uint32_t i = tid; i < r_n; i += kThreads) {
+            Key<KW, NC> k = load_key<KW, NC>(t.r, r_lo + i);
+#pragma unroll
+            for (int c = 0; c < NC; c++) rk[c * CAP + i] = k.c[c];
+            rid[i] = t.r.rowids ? t.r.rowids[r_lo + i] : r_lo + i;
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < r_n; i += kThreads) {
+            Key<KW, NC> k;
+#pragma unroll
+            for (int c = 0; c < NC; c++) k.c[c] = rk[c * CAP + i];
+            const uint64_t h = hash_key<HP, KW, NC>(k, hc);
+            const uint32_t tag = (uint32_t)(h >> tag_shift) & 0xFFFFu;
+            const uint32_t e = (tag << 16) | (i + 1);
+            uint32_t sl = (uint32_t)(h >> shift) & smask;
+            while (true) {
+                const uint32_t old = atomicCAS(&slots[sl], 0u, e);
+                if (old == 0) break;
+                if ((old >> 16) == tag) {
+                    const uint32_t o = (old & 0xFFFFu) - 1;
+                    bool eq = true;
+#pragma unroll
+                    for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == k.c[c];
+                    if (eq) t.dup = 1;
+                }
+                sl = (sl + 1) & smask;
+            }
+        }
+        __syncthreads();
+
+        // probe: each wave walks 64 * kProbeUnroll rows of the S chunk per step
+        const bool dup = t.dup != 0;
+        const uint32_t s_lo = t.s_lo, s_end = t.s_lo + t.s_n, ob1 = t.one_based;
+        uint32_t wn = 0;
+        for (uint32_t base = s_lo + w * 64 * kProbeUnroll; base < s_end; base += kThreads * kProbeUnroll) {
+            Key<KW, NC> sk[kProbeUnroll];
+            uint32_t sid[kProbeUnroll];
+#pragma unroll
+            for (int u = 0; u < kProbeUnroll; u++) {
+                const uint32_t i = base + u * 64 + lane;
+                if (i < s_end) {
+                    sk[u] = load_key<KW, NC>(t.s, i);
+                    sid[u] = t.s.rowids ? t.s.rowids[i] : i;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kProbeUnroll; u++) {
+                const uint32_t i = base + u * 64 + lane;
+                bool active = i < s_end;
+                uint32_t sl = 0, tag = 0;
+                if (active) {
+                    const uint64_t h = hash_key<HP, KW, NC>(sk[u], hc);
+                    tag = (uint32_t)(h >> tag_shift) & 0xFFFFu;
+                    sl = (uint32_t)(h >> shift) & smask;
+                }
+                while (__ballot(active)) {
+                    bool m = false;
+                    uint32_t rrow = 0;
+                    if (active) {
+                        const uint32_t e = slots[sl];
+                        if (e == 0) {
+                            active = false;
+                        } else {
+                            if ((e >> 16) == tag) {
+                                const uint32_t o = (e & 0xFFFFu) - 1;
+                                bool eq = true;
+#pragma unroll
+                                for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == sk[u].c[c];
+                                if (eq) {
+                                    m = true;
+                                    rrow = rid[o];
+                                    if (!dup) active = false;
+                                }
+                            }
+                            sl = (sl + 1) & smask;
+                        }
+                    }
+                    const uint64_t mb = __ballot(m);
+                    if (mb) {
+                        const uint32_t c = __builtin_popcountll(mb);
+                        if (wn + c > (uint32_t)kOutBuf) flush_wave<KW, NC>(obr, obs, wn, t);
+                        if (m) {
+                            const uint32_t p = wn + lanes_below(mb);
+                            obr[p] = rrow + ob1;
+                            obs[p] = sid[u] + ob1;
+                        }
+                        wn += c;
+                    }
+                }
+            }
+        }
+        if (wn) flush_wave<KW, NC>(obr, obs, wn, t);
+        __syncthreads();
+    }
+}
+
+__global__ void k_emit_hist(const uint32_t *base, uint32_t nb, uint32_t *hist, uint32_t *offsets) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) {
+        offsets[i] = base[i];
+        hist[i] = base[i + 1] - base[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch over (hash path, key width, key columns)
+// ---------------------------------------------------------------------------
+
+template <typename F> hipError_t dispatch(const LaunchShape &sh, F &&f) {
+#define VJ_CASE(HP, KW, NC) \
+    if (sh.hp == HP && sh.kw == KW && sh.nc == NC) return f.template operator()<HP, KW, NC>();
+#define VJ_CASES_HP(HP) \
+    VJ_CASE(HP, 4, 1) VJ_CASE(HP, 4, 2) VJ_CASE(HP, 4, 3) VJ_CASE(HP, 4, 4) \
+    VJ_CASE(HP, 8, 1) VJ_CASE(HP, 8, 2) VJ_CASE(HP, 8, 3) VJ_CASE(HP, 8, 4)
+    VJ_CASES_HP(HP_FNV)
+    VJ_CASES_HP(HP_XOR_HW)
+    VJ_CASES_HP(HP_GENERIC)
+#undef VJ_CASES_HP
+#undef VJ_CASE
+    return hipErrorInvalidValue;
+}
+
+size_t pass_lds_bytes(int kw, int nc) {
+    const int kb = kw * nc, T = pass_tile(kb);
+    return (size_t)T * kb + (size_t)T * 4 + (size_t)(kWaves * 256 + 512) * 4 + (size_t)T;
+}
+
+size_t probe_lds_bytes(int kw, int nc) {
+    const int kb = kw * nc, cap = build_cap(kb);
+    return (size_t)cap * kb + (size_t)cap * 4 + (size_t)cap * 8 + (size_t)kWaves * 2 * kOutBuf * 4;
+}
+
+}  // namespace
+
+hipError_t launch_prepare(const PipeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_hist(const PipeArgs &a, const LaunchShape &sh, hipStream_t s) {
+    if (a.total_hist_blocks == 0) return hipSuccess;
+    const size_t lds = (size_t)a.max_hist_bins * 4;
+    return dispatch(sh, [&]<int HP, int KW, int NC>() {
+        hipLaunchKernelGGL((k_hist<HP, KW, NC>), dim3(a.total_hist_blocks), dim3(kHistThreads), lds, s, a);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_scan(const PipeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_scan, dim3(a.n_jobs), dim3(1024), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (a.n_joins) {
+        hipLaunchKernelGGL(k_tasks, dim3(a.n_joins), dim3(1024), 0, s, a);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_probe(const PipeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_probe, dim3(1), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass(const PipeArgs &a, const LaunchShape &sh, int pass, hipStream_t s) {
+    const uint32_t grid = pass == 1 ? a.total_p1_tiles : a.total_p2_tiles_max;
+    if (grid == 0) return hipSuccess;
+    const size_t lds = pass_lds_bytes(sh.kw, sh.nc);
+    return dispatch(sh, [&]<int HP, int KW, int NC>() {
+        hipLaunchKernelGGL((k_pass<HP, KW, NC>), dim3(grid), dim3(kThreads), lds, s, a, pass);
+        return hipGetLastError();
+    });
+}
+
+int probe_blocks_per_cu(const LaunchShape &sh) {
+    const size_t lds = probe_lds_bytes(sh.kw, sh.nc) + sizeof(ProbeTask);
+    int by_lds = (int)((160 * 1024) / lds);
+    return by_lds < 1 ? 1 : (by_lds > 8 ? 8 : by_lds);
+}
+
+hipError_t launch_probe(const PipeArgs &a, const LaunchShape &sh, int blocks, hipStream_t s) {
+    const size_t lds = probe_lds_bytes(sh.kw, sh.nc);
+    return dispatch(sh, [&]<int HP, int KW, int NC>() {
+        hipLaunchKernelGGL((k_probe<HP, KW, NC>), dim3(blocks), dim3(kThreads), lds, s, a);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_emit_hist(const uint32_t *base, uint32_t nb, uint32_t *hist, uint32_t *offsets, hipStream_t s) {
+    unsigned blocks = (nb + 255) / 256;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_emit_hist, dim3(blocks), dim3(256), 0, s, base, nb, hist, offsets);
+    return hipGetLastError();
+}
+
+}  // namespace vj

@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark of the single-GPU hash join (one MI355X).
+
+A step is one complete join of the large FK workload: build 2^24 unique int64
+keys, probe 2^27 int64 keys drawn from the build keys (100 % match), FNV-1a-64,
+2 radix passes (bits chosen by the planner), all (build_rowid, probe_rowid)
+pairs written to HBM. Inputs are resident in HBM before the timed region; the
+output buffers are allocated before it too (allocation is timed separately).
+
+Prints one JSON line. `value` is probe tuples per second of the whole join.
+`roofline` is the dominant kernel's algorithmic bytes per launch over its
+average duration, from HIP events the library records on the join's stream
+around each stage. `cpu_baseline` is the oracle's OpenMP implementation of the
+same algorithm on the host cores, on a bounded sample of the workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--log-r 24 --log-s 27]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("vectorized-join_amd", "oracle"):
+    sys.path.insert(0, os.path.join(ROOT, sub))
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+KEY_BYTES = 8
+ROWID_BYTES = 4
+PAIR_BYTES = 8
+
+
+def algorithmic_bytes(nr, ns, m, passes):
+    """B = (2P + 1) * (|R| w + |S| w) + M * 2 * 4 with w = 8 B key + 4 B row id."""
+    w = KEY_BYTES + ROWID_BYTES
+    return (2 * passes + 1) * (nr + ns) * w + m * PAIR_BYTES
+
+
+def stage_bytes(nr, ns, m):
+    """Bytes each stage must move (its algorithmic traffic)."""
+    n = nr + ns
+    return {
+        "histogram": n * KEY_BYTES,                                   # read keys
+        "pass1": n * (KEY_BYTES + KEY_BYTES + ROWID_BYTES),            # read keys, write key + row id
+        "pass2": n * 2 * (KEY_BYTES + ROWID_BYTES),                   # read + write key + row id
+        "build_probe": n * (KEY_BYTES + ROWID_BYTES) + m * PAIR_BYTES,  # read tuples, write pairs
+    }
+
+
+def cpu_baseline(log_r, log_s, passes, bits1, bits2):
+    import numpy as np
+
+    import oracle
+    import workloads
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    rcols, scols = workloads.large_fk(log_r=log_r, log_s=log_s)
+    r = rcols[0].view(np.uint64).reshape(-1, 1)
+    s = scols[0].view(np.uint64).reshape(-1, 1)
+    cap = len(s)
+    oracle.join(r[:4096], s[:4096], passes=1, bits1=4, nthreads=threads, cap=cap)  # warm-up
+    t0 = time.perf_counter()
+    rr, ss = oracle.join(r, s, "fnv", 64, 64, passes=passes, bits1=bits1, bits2=bits2, load=0.25, nthreads=threads,
+                         cap=cap)
+    dt = time.perf_counter() - t0
+    assert len(rr) == len(s)
+    return {"value": len(s) / dt, "unit": "probe tuples/s", "cores": threads, "kind": "port",
+            "sample": "large_fk at build 2^%d, probe 2^%d (FNV-1a-64, %d passes, bits %d+%d, OpenMP oracle "
+                      "oracle/vj_oracle.c, %.2f s)" % (log_r, log_s, passes, bits1, bits2, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log-r", type=int, default=24)
+    ap.add_argument("--log-s", type=int, default=27)
+    ap.add_argument("--passes", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-log-r", type=int, default=23)
+    ap.add_argument("--cpu-log-s", type=int, default=26)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the join shards by a top radix digit across GPUs "
+              "(see DESIGN.md) but no multi-GPU run exists. Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import vjoin
+    import workloads
+
+    torch.cuda.set_device(0)
+    print("bench: generating large_fk build 2^%d probe 2^%d" % (args.log_r, args.log_s), file=sys.stderr, flush=True)
+    rcols, scols = workloads.large_fk(log_r=args.log_r, log_s=args.log_s)
+    nr, ns = len(rcols[0]), len(scols[0])
+    dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+    del rcols, scols
+    ctx = vjoin.Context(0)
+    ctx.set_profiling(True)
+    opts = vjoin.join_opts("fnv", 64, 64, passes=args.passes)
+    build, probe = vjoin.table(dr), vjoin.table(ds)
+    t_alloc = time.perf_counter()
+    out_r = torch.empty(ns, dtype=torch.int32, device="cuda")
+    out_s = torch.empty(ns, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    t_alloc = time.perf_counter() - t_alloc
+
+    def step():
+        rc, cnt = ctx.join_into(build, probe, opts, out_r, out_s)
+        if rc != 0:
+            raise RuntimeError("join failed: %s %s" % (vjoin.STATUS.get(rc), ctx.last_error()))
+        return cnt
+
+    t0 = time.perf_counter()
+    for _ in range(args.warmup):
+        m = step()
+    t_warm = time.perf_counter() - t0
+    print("bench: warmup done (%.2f s), first-join scratch %.2f GB" % (t_warm, ctx.memory() / 1e9), file=sys.stderr,
+          flush=True)
+    # correctness of the last warm-up output (size-independent property of an FK join)
+    sid = out_s.long()
+    ok = bool(torch.equal(torch.sort(sid).values, torch.arange(ns, device="cuda"))) and \
+        bool(torch.equal(dr[0][out_r.long()], ds[0][sid]))
+    del sid
+    stages = {"histogram": [], "pass1": [], "pass2": [], "build_probe": [], "total": []}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m = step()
+        s = ctx.summary()
+        stages["total"].append(s["ms_total"])
+        stages["histogram"].append(s["ms_histogram"])
+        stages["pass1"].append(s["ms_pass1"])
+        stages["pass2"].append(s["ms_pass2"])
+        stages["build_probe"].append(s["ms_build_probe"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    summ = ctx.summary()
+    ms_step = elapsed / args.steps * 1e3
+    med = {k: float(np.median(v)) for k, v in stages.items()}
+    sb = stage_bytes(nr, ns, m)
+    dominant = max(("histogram", "pass1", "pass2", "build_probe"), key=lambda k: med[k])
+    achieved = sb[dominant] / (float(np.mean(stages[dominant])) * 1e-3)
+    B = algorithmic_bytes(nr, ns, m, summ["passes"])
+    join_gbs = B / (med["total"] * 1e-3)
+    out = {
+        "metric": "probe tuples/s + HBM GB/s (frac of peak); batched vs plain joins/s, p99 latency",
+        "value": ns / (ms_step * 1e-3),
+        "unit": "probe tuples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (seeded: workloads.large_fk)",
+        "config": {"workload": "large_fk: build 2^%d unique int64 keys (mix64 of a permutation), probe 2^%d FK "
+                               "keys (100%% match), FNV-1a-64, %d radix passes" % (args.log_r, args.log_s,
+                                                                                   summ["passes"]),
+                   "build_rows": nr, "probe_rows": ns, "matches": m, "passes": summ["passes"],
+                   "bits": [summ["bits1"], summ["bits2"]], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "algorithmic_bytes_per_launch": sb[dominant]},
+        "join_roofline": {"algorithmic_bytes": B, "ms": med["total"], "achieved_GBps": join_gbs / 1e9,
+                          "frac": join_gbs / HBM_PEAK},
+        "stage_ms_median": med,
+        "stage_GBps": {k: sb[k] / (med[k] * 1e-3) / 1e9 for k in sb if med[k] > 0},
+        "output_alloc_ms": t_alloc * 1e3,
+        "parity": {"fk_every_probe_row_once_equal_keys": ok},
+    }
+    if not args.no_cpu:
+        print("bench: cpu baseline (OpenMP oracle)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_log_r, args.cpu_log_s, summ["passes"], summ["bits1"],
+                                           summ["bits2"])
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -21,6 +21,7 @@ namespace {
 
 constexpr size_t kAlign = 256;
 constexpr uint32_t kDefaultCapS = 16384;  // probe rows per build+probe task
+constexpr uint32_t kMaxSegments = 256;   // pass-1 look-back sequences per job
 
 inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
@@ -40,7 +41,7 @@ struct vj_ctx {
     bool profiling = false;
     std::string last_error;
     vj_join_summary summary{};
-    DevBuf tuples, hist, base, p2pre, ptask, status, meta;
+    DevBuf tuples, hist, base, p2pre, ptask, status, segs, meta;
     bool hist_dirty = true;
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
@@ -53,7 +54,7 @@ namespace {
 
 uint64_t held_bytes(const vj_ctx *c) {
     return c->tuples.bytes + c->hist.bytes + c->base.bytes + c->p2pre.bytes + c->ptask.bytes + c->status.bytes +
-           c->meta.bytes;
+           c->segs.bytes + c->meta.bytes;
 }
 
 void free_buf(DevBuf &b) {
@@ -69,6 +70,7 @@ void free_all(vj_ctx *c) {
     free_buf(c->p2pre);
     free_buf(c->ptask);
     free_buf(c->status);
+    free_buf(c->segs);
     free_buf(c->meta);
     c->hist_dirty = true;
 }
@@ -112,6 +114,7 @@ struct Plan {
     std::vector<size_t> fin_off, mid_off;  // byte offsets in the tuples arena (SIZE_MAX: none)
     size_t tuples_bytes = 0;
     uint64_t hist_words = 0, base_words = 0, p2pre_words = 0, ptask_words = 0, status_words = 0;
+    uint64_t seg_words = 0, segc_words = 0;  // per-segment tickets; (digit, segment) counts
     uint32_t max_hist_bins = 1, total_hist_blocks = 0, total_p1_tiles = 0, total_p2_tiles_max = 0;
     bool any_part = false, any_p2 = false;
     LaunchShape shape{};
@@ -120,6 +123,7 @@ struct Plan {
     // meta layout
     size_t off_jobs = 0, off_joins = 0, off_hb0 = 0, off_pt0 = 0, upload_bytes = 0;
     size_t off_jp2 = 0, off_p2t0 = 0, off_jtasks = 0, off_task0 = 0, off_count = 0, off_ctl = 0, meta_bytes = 0;
+    size_t off_segt = 0, off_p2t = 0, zero_begin = 0, zero_end = 0;
 };
 
 int ceil_log2(uint64_t v) {
@@ -183,8 +187,14 @@ void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0
         jd.hist_off = (uint32_t)pl.hist_words;
         pl.hist_words += nb;
         pl.max_hist_bins = std::max(pl.max_hist_bins, nb);
-        jd.hist_blocks = jd.n ? (uint32_t)std::min<uint64_t>(512, std::max<uint64_t>(1, ((uint64_t)jd.n + 65535) / 65536)) : 0;
         jd.p1_tiles = (uint32_t)(((uint64_t)jd.n + T - 1) / T);
+        // pass-1 segments (= histogram blocks): at most kMaxSegments per job
+        jd.seg_tiles = jd.p1_tiles ? (jd.p1_tiles + kMaxSegments - 1) / kMaxSegments : 1;
+        jd.hist_blocks = jd.p1_tiles ? (jd.p1_tiles + jd.seg_tiles - 1) / jd.seg_tiles : 0;
+        jd.seg_off = (uint32_t)pl.seg_words;
+        pl.seg_words += jd.hist_blocks;
+        jd.segc_off = (uint32_t)pl.segc_words;
+        pl.segc_words += (uint64_t)jd.hist_blocks << jd.b1;
         jd.p1_stat_off = pl.status_words;
         pl.status_words += (uint64_t)jd.p1_tiles << jd.b1;
         if (jd.b2) {
@@ -230,16 +240,23 @@ void finish_layout(Plan &pl) {
     pl.off_pt0 = o;
     o = align_up(o + (nj + 1) * 4);
     pl.upload_bytes = o;
+    // counters cleared by k_prepare: one contiguous region
+    pl.zero_begin = o;
     pl.off_jp2 = o;
     o = align_up(o + nj * 4);
-    pl.off_p2t0 = o;
-    o = align_up(o + (nj + 1) * 4);
     pl.off_jtasks = o;
     o = align_up(o + nn * 8);
-    pl.off_task0 = o;
-    o = align_up(o + (nn + 1) * 8);
     pl.off_count = o;
     o = align_up(o + nn * 8);
+    pl.off_segt = o;
+    o = align_up(o + pl.seg_words * 4);
+    pl.off_p2t = o;
+    o = align_up(o + pl.p2pre_words * 4);
+    pl.zero_end = o;
+    pl.off_p2t0 = o;
+    o = align_up(o + (nj + 1) * 4);
+    pl.off_task0 = o;
+    o = align_up(o + (nn + 1) * 8);
     pl.off_ctl = o;
     o = align_up(o + sizeof(Ctl));
     pl.meta_bytes = o;
@@ -287,7 +304,7 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
 uint64_t plan_scratch_bytes(const Plan &pl) {
     return align_up(pl.tuples_bytes) + align_up(pl.hist_words * 4) + align_up(pl.base_words * 4) +
            align_up(pl.p2pre_words * 4) + align_up(pl.ptask_words * 8) + align_up(pl.status_words * 8) +
-           align_up(pl.meta_bytes);
+           align_up(pl.segc_words * 8) + align_up(pl.meta_bytes);
 }
 
 // Admission (DeviceConfig::is_out_of_memory, join_provider.cuh:148-150): the
@@ -303,7 +320,8 @@ Err admit(vj_ctx *c, const Plan &pl, uint64_t extra) {
     keep += std::max<uint64_t>(c->tuples.bytes, pl.tuples_bytes) + std::max<uint64_t>(c->hist.bytes, pl.hist_words * 4) +
             std::max<uint64_t>(c->base.bytes, pl.base_words * 4) + std::max<uint64_t>(c->p2pre.bytes, pl.p2pre_words * 4) +
             std::max<uint64_t>(c->ptask.bytes, pl.ptask_words * 8) +
-            std::max<uint64_t>(c->status.bytes, pl.status_words * 8) + std::max<uint64_t>(c->meta.bytes, pl.meta_bytes);
+            std::max<uint64_t>(c->status.bytes, pl.status_words * 8) + std::max<uint64_t>(c->segs.bytes, pl.segc_words * 8) +
+            std::max<uint64_t>(c->meta.bytes, pl.meta_bytes);
     if (keep + extra > c->budget) free_all(c);
     return Err::ok();
 }
@@ -320,6 +338,7 @@ Err allocate(vj_ctx *c, Plan &pl) {
     VJ_TRY(ensure(c->p2pre, std::max<size_t>(pl.p2pre_words * 4, kAlign)));
     VJ_TRY(ensure(c->ptask, std::max<size_t>(pl.ptask_words * 8, kAlign)));
     VJ_TRY(ensure(c->status, std::max<size_t>(pl.status_words * 8, kAlign)));
+    VJ_TRY(ensure(c->segs, std::max<size_t>(pl.segc_words * 8, kAlign)));
     VJ_TRY(ensure(c->meta, pl.meta_bytes));
     VJ_TRY(ensure_pinned(c, pl.meta_bytes));
     // tuple buffer pointers
@@ -355,6 +374,12 @@ PipeArgs make_args(vj_ctx *c, const Plan &pl) {
     a.task0 = reinterpret_cast<unsigned long long *>(m + pl.off_task0);
     a.out_count = reinterpret_cast<unsigned long long *>(m + pl.off_count);
     a.ctl = reinterpret_cast<Ctl *>(m + pl.off_ctl);
+    a.seg_ticket = reinterpret_cast<uint32_t *>(m + pl.off_segt);
+    a.p2_ticket = reinterpret_cast<uint32_t *>(m + pl.off_p2t);
+    a.segcnt = static_cast<uint32_t *>(c->segs.p);
+    a.segbase = static_cast<uint32_t *>(c->segs.p) + pl.segc_words;
+    a.zero_begin = m + pl.zero_begin;
+    a.zero_words = (pl.zero_end - pl.zero_begin) / 4;
     a.hist = static_cast<uint32_t *>(c->hist.p);
     a.base = static_cast<uint32_t *>(c->base.p);
     a.p2pre = static_cast<uint32_t *>(c->p2pre.p);

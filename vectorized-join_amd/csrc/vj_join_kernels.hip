@@ -124,18 +124,15 @@ template <typename T> __device__ void block_scan_array(const T *in, T *out, uint
 // ---------------------------------------------------------------------------
 
 __global__ void k_prepare(PipeArgs a) {
-    if (threadIdx.x == 0) {
-        a.ctl->p1_ticket = 0;
-        a.ctl->p2_ticket = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctl->p2_total = 0;
         a.ctl->task_ticket = 0;
         a.ctl->task_total = 0;
     }
-    for (uint32_t j = threadIdx.x; j < a.n_joins; j += blockDim.x) {
-        a.out_count[j] = 0;
-        a.join_tasks[j] = 0;
-    }
-    for (uint32_t j = threadIdx.x; j < a.n_jobs; j += blockDim.x) a.job_p2_tiles[j] = 0;
+    uint32_t *z = static_cast<uint32_t *>(a.zero_begin);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.zero_words;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        z[i] = 0;
 }
 
 constexpr int kHistThreads = 1024;
@@ -155,9 +152,10 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(PipeArgs a) {
     const uint32_t nb = 1u << (job.b1 + job.b2);
     for (uint32_t i = threadIdx.x; i < nb; i += kHistThreads) lds_hist[i] = 0;
     __syncthreads();
-    const uint32_t k = b - a.hist_block0[j];
-    const uint64_t lo = (uint64_t)job.n * k / job.hist_blocks;
-    const uint64_t hi = (uint64_t)job.n * (k + 1) / job.hist_blocks;
+    const uint32_t k = b - a.hist_block0[j];  // segment of this block
+    const uint64_t seg_rows = (uint64_t)job.seg_tiles * a.pass_tile;
+    const uint64_t lo = (uint64_t)k * seg_rows;
+    const uint64_t hi = lo + seg_rows < job.n ? lo + seg_rows : job.n;
     const uint32_t m1 = (1u << job.b1) - 1, m2 = (1u << job.b2) - 1;
     const uint32_t s1 = job.shift0, s2 = job.shift0 + job.b1;
     const HashCfg hc = a.hash;
@@ -185,6 +183,13 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(PipeArgs a) {
     for (uint32_t t = threadIdx.x; t < nb; t += kHistThreads) {
         uint32_t c = lds_hist[t];
         if (c) atomicAdd(&g[t], c);
+    }
+    // pass-1 digit counts of this segment (digit-major: [d1][segment])
+    const uint32_t n1 = 1u << job.b1, n2 = 1u << job.b2;
+    for (uint32_t d1 = threadIdx.x; d1 < n1; d1 += kHistThreads) {
+        uint32_t c = 0;
+        for (uint32_t d2 = 0; d2 < n2; d2++) c += lds_hist[(d1 << job.b2) | d2];
+        a.segcnt[job.segc_off + d1 * job.hist_blocks + k] = c;
     }
 }
 
@@ -215,6 +220,23 @@ __global__ __launch_bounds__(1024) void k_scan(PipeArgs a) {
         __syncthreads();
         block_scan_array<uint32_t>(pre, pre, n1, sh32);
         if (threadIdx.x == 0) a.job_p2_tiles[jj] = pre[n1];
+    }
+    // output base of every (pass-1 digit, segment): one wave scans one digit's segments
+    const uint32_t nseg = job.hist_blocks, n1 = 1u << job.b1;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const uint32_t per = (nseg + 63) / 64;
+    for (uint32_t d1 = wv; d1 < n1; d1 += nwv) {
+        const uint32_t *cnt = a.segcnt + job.segc_off + d1 * nseg;
+        uint32_t *sb = a.segbase + job.segc_off + d1 * nseg;
+        const uint32_t lo = lane * per < nseg ? lane * per : nseg, hi = lo + per < nseg ? lo + per : nseg;
+        uint32_t sum = 0;
+        for (uint32_t g = lo; g < hi; g++) sum += cnt[g];
+        uint32_t pre = wave_incl_scan(sum) - sum + base[d1 << job.b2];
+        for (uint32_t g = lo; g < hi; g++) {
+            const uint32_t c = cnt[g];
+            sb[g] = pre;
+            pre += c;
+        }
     }
 }
 
@@ -292,43 +314,47 @@ __global__ __launch_bounds__(kThreads) void k_pass(PipeArgs a, int pass) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     for (uint32_t i = tid; i < kWaves * 256; i += kThreads) wcnt[i] = 0;
     if (w == 0) {
-        uint32_t ticket = 0;
-        if (lane == 0) ticket = atomicAdd(pass == 1 ? &a.ctl->p1_ticket : &a.ctl->p2_ticket, 1u);
-        ticket = __shfl(ticket, 0, 64);
+        // Blocks map statically onto look-back sequences (pass 1: a segment of a
+        // job; pass 2: a pass-1 partition of a job), as many blocks as the
+        // sequence has tiles; inside a sequence the order comes from that
+        // sequence's own ticket, so a tile only waits on tiles whose blocks
+        // already run, and no counter is shared by more than one sequence.
+        const uint32_t b = blockIdx.x;
         uint32_t valid, j, local;
         if (pass == 1) {
-            valid = ticket < a.total_p1_tiles;
-            j = valid ? wave_upper_index(a.p1_tile0, a.n_jobs, ticket) : 0;
-            local = valid ? ticket - a.p1_tile0[j] : 0;
+            valid = b < a.total_p1_tiles;
+            j = valid ? wave_upper_index(a.p1_tile0, a.n_jobs, b) : 0;
+            local = valid ? b - a.p1_tile0[j] : 0;
         } else {
-            valid = ticket < a.ctl->p2_total;
-            j = valid ? wave_upper_index(a.p2_tile0, a.n_jobs, ticket) : 0;
-            local = valid ? ticket - a.p2_tile0[j] : 0;
+            valid = b < a.ctl->p2_total;
+            j = valid ? wave_upper_index(a.p2_tile0, a.n_jobs, b) : 0;
+            local = valid ? b - a.p2_tile0[j] : 0;
         }
-        uint32_t d1 = 0, lt = local;
         const JobDesc &job = a.jobs[j];
         const uint32_t *pre = a.p2pre + job.p2pre_off;
-        if (valid && pass == 2) {
-            d1 = wave_upper_index(pre, 1u << job.b1, local);
-            lt = local - pre[d1];
-        }
+        uint32_t seq = 0;  // segment (pass 1) or pass-1 partition (pass 2)
+        if (valid) seq = pass == 1 ? local / job.seg_tiles : wave_upper_index(pre, 1u << job.b1, local);
         if (lane == 0) {
             ti.valid = valid;
             if (valid) {
                 const uint32_t *base = a.base + job.base_off;
                 if (pass == 1) {
+                    const uint32_t lt = atomicAdd(&a.seg_ticket[job.seg_off + seq], 1u);
+                    const uint32_t tile = seq * job.seg_tiles + lt;
                     ti.src = job.in;
                     for (int c = 0; c < 4; c++) ti.dst_cols[c] = job.b2 ? job.mid_cols[c] : job.fin_cols[c];
                     ti.dst_ids = job.b2 ? job.mid_ids : job.fin_ids;
-                    ti.dbase = base;
-                    ti.dstride = 1u << job.b2;
-                    ti.lo = local * T;
+                    ti.dbase = a.segbase + job.segc_off + seq;
+                    ti.dstride = job.hist_blocks;
+                    ti.lo = tile * T;
                     ti.hi = ti.lo + T < job.n ? ti.lo + T : job.n;
                     ti.bins = 1u << job.b1;
                     ti.shift = job.shift0;
-                    ti.seq = a.status + job.p1_stat_off;
-                    ti.local = local;
+                    ti.seq = a.status + job.p1_stat_off + (uint64_t)seq * job.seg_tiles * (1u << job.b1);
+                    ti.local = lt;
                 } else {
+                    const uint32_t d1 = seq;
+                    const uint32_t lt = atomicAdd(&a.p2_ticket[job.p2pre_off + d1], 1u);
                     for (int c = 0; c < 4; c++) ti.src.cols[c] = job.mid_cols[c];
                     ti.src.rowids = job.mid_ids;
                     ti.src.n = job.n;
@@ -450,22 +476,22 @@ struct ProbeTask {
     uint32_t *out_r, *out_s;
     unsigned long long *count;
     uint64_t cap;
+    unsigned long long out_base;  // first output row of this task
     uint32_t r_lo, r_n, s_lo, s_n;
-    uint32_t nslots, shift, one_based, tag_shift;
+    uint32_t nslots, shift, one_based;
     uint32_t dup;
     uint32_t done;
+    uint32_t wave_count[kWaves];
 };
 
 constexpr int kProbeUnroll = 8;
 
-template <int KW, int NC>
-__device__ __forceinline__ void flush_wave(uint32_t *obr, uint32_t *obs, uint32_t &wn, const ProbeTask &t) {
+// Writes the wave's staged pairs to rows [pos, pos + wn) of the join output.
+__device__ __forceinline__ void flush_wave(const uint32_t *obr, const uint32_t *obs, uint32_t wn,
+                                           unsigned long long pos, const ProbeTask &t) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const uint32_t lane = lane_id();
-    unsigned long long pos = 0;
-    if (lane == 0) pos = atomicAdd(t.count, (unsigned long long)wn);
-    pos = __shfl(pos, 0, 64);
     for (uint32_t k = lane; k < wn; k += 64) {
         const unsigned long long o = pos + k;
         if (o < t.cap) {
@@ -474,7 +500,89 @@ __device__ __forceinline__ void flush_wave(uint32_t *obr, uint32_t *obs, uint32_
         }
     }
     __builtin_amdgcn_wave_barrier();
-    wn = 0;
+}
+
+// Probes this wave's rows of the S chunk against the LDS table. EMIT = false
+// only counts matches; EMIT = true stages them in the wave's LDS buffer and
+// writes them from row `pos` on. Both walks visit rows and table slots in the
+// same order, so the counts of the first walk are the row counts of the second.
+template <bool EMIT, int HP, int KW, int NC>
+__device__ __forceinline__ uint32_t probe_walk(const ProbeTask &t, const HashCfg &hc, uint32_t tag_shift,
+                                               const typename KeyT<KW>::T *rk, const uint32_t *rid,
+                                               const uint32_t *slots, uint32_t *obr, uint32_t *obs,
+                                               unsigned long long pos) {
+    constexpr int CAP = build_cap(KW * NC);
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t smask = t.nslots - 1, shift = t.shift, ob1 = t.one_based;
+    const bool dup = t.dup != 0;
+    const uint32_t s_end = t.s_lo + t.s_n;
+    uint32_t total = 0, wn = 0;
+    for (uint32_t base = t.s_lo + w * 64 * kProbeUnroll; base < s_end; base += kThreads * kProbeUnroll) {
+        Key<KW, NC> sk[kProbeUnroll];
+        uint32_t sid[kProbeUnroll];
+#pragma unroll
+        for (int u = 0; u < kProbeUnroll; u++) {
+            const uint32_t i = base + u * 64 + lane;
+            if (i < s_end) {
+                sk[u] = load_key<KW, NC>(t.s, i);
+                if (EMIT) sid[u] = t.s.rowids ? t.s.rowids[i] : i;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kProbeUnroll; u++) {
+            const uint32_t i = base + u * 64 + lane;
+            bool active = i < s_end;
+            uint32_t sl = 0, tag = 0;
+            if (active) {
+                const uint64_t h = hash_key<HP, KW, NC>(sk[u], hc);
+                tag = (uint32_t)(h >> tag_shift) & 0xFFFFu;
+                sl = (uint32_t)(h >> shift) & smask;
+            }
+            while (__ballot(active)) {
+                bool m = false;
+                uint32_t rrow = 0;
+                if (active) {
+                    const uint32_t e = slots[sl];
+                    if (e == 0) {
+                        active = false;
+                    } else {
+                        if ((e >> 16) == tag) {
+                            const uint32_t o = (e & 0xFFFFu) - 1;
+                            bool eq = true;
+#pragma unroll
+                            for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == sk[u].c[c];
+                            if (eq) {
+                                m = true;
+                                if (EMIT) rrow = rid[o];
+                                if (!dup) active = false;
+                            }
+                        }
+                        sl = (sl + 1) & smask;
+                    }
+                }
+                const uint64_t mb = __ballot(m);
+                if (mb) {
+                    const uint32_t c = __builtin_popcountll(mb);
+                    if (EMIT) {
+                        if (wn + c > (uint32_t)kOutBuf) {
+                            flush_wave(obr, obs, wn, pos, t);
+                            pos += wn;
+                            wn = 0;
+                        }
+                        if (m) {
+                            const uint32_t p = wn + lanes_below(mb);
+                            obr[p] = rrow + ob1;
+                            obs[p] = sid[u] + ob1;
+                        }
+                        wn += c;
+                    }
+                    total += c;
+                }
+            }
+        }
+    }
+    if (EMIT && wn) flush_wave(obr, obs, wn, pos, t);
+    return total;
 }
 
 template <int HP, int KW, int NC>
@@ -582,68 +690,21 @@ __global__ __launch_bounds__(kThreads) void k_probe(PipeArgs a) {
         }
         __syncthreads();
 
-        // probe: each wave walks 64 * kProbeUnroll rows of the S chunk per step
-        const bool dup = t.dup != 0;
-        const uint32_t s_lo = t.s_lo, s_end = t.s_lo + t.s_n, ob1 = t.one_based;
-        uint32_t wn = 0;
-        for (uint32_t base = s_lo + w * 64 * kProbeUnroll; base < s_end; base += kThreads * kProbeUnroll) {
-            Key<KW, NC> sk[kProbeUnroll];
-            uint32_t sid[kProbeUnroll];
-#pragma unroll
-            for (int u = 0; u < kProbeUnroll; u++) {
-                const uint32_t i = base + u * 64 + lane;
-                if (i < s_end) {
-                    sk[u] = load_key<KW, NC>(t.s, i);
-                    sid[u] = t.s.rowids ? t.s.rowids[i] : i;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kProbeUnroll; u++) {
-                const uint32_t i = base + u * 64 + lane;
-                bool active = i < s_end;
-                uint32_t sl = 0, tag = 0;
-                if (active) {
-                    const uint64_t h = hash_key<HP, KW, NC>(sk[u], hc);
-                    tag = (uint32_t)(h >> tag_shift) & 0xFFFFu;
-                    sl = (uint32_t)(h >> shift) & smask;
-                }
-                while (__ballot(active)) {
-                    bool m = false;
-                    uint32_t rrow = 0;
-                    if (active) {
-                        const uint32_t e = slots[sl];
-                        if (e == 0) {
-                            active = false;
-                        } else {
-                            if ((e >> 16) == tag) {
-                                const uint32_t o = (e & 0xFFFFu) - 1;
-                                bool eq = true;
-#pragma unroll
-                                for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == sk[u].c[c];
-                                if (eq) {
-                                    m = true;
-                                    rrow = rid[o];
-                                    if (!dup) active = false;
-                                }
-                            }
-                            sl = (sl + 1) & smask;
-                        }
-                    }
-                    const uint64_t mb = __ballot(m);
-                    if (mb) {
-                        const uint32_t c = __builtin_popcountll(mb);
-                        if (wn + c > (uint32_t)kOutBuf) flush_wave<KW, NC>(obr, obs, wn, t);
-                        if (m) {
-                            const uint32_t p = wn + lanes_below(mb);
-                            obr[p] = rrow + ob1;
-                            obs[p] = sid[u] + ob1;
-                        }
-                        wn += c;
-                    }
-                }
-            }
+        // count, reserve the task's output rows with one atomic, then write
+        const uint32_t wc = probe_walk<false, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, obr, obs, 0);
+        if (lane == 0) t.wave_count[w] = wc;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t sum = 0;
+            for (int k = 0; k < kWaves; k++) sum += t.wave_count[k];
+            t.out_base = sum ? atomicAdd(t.count, (unsigned long long)sum) : 0;
         }
-        if (wn) flush_wave<KW, NC>(obr, obs, wn, t);
+        __syncthreads();
+        if (wc) {
+            unsigned long long pos = t.out_base;
+            for (uint32_t k = 0; k < w; k++) pos += t.wave_count[k];
+            probe_walk<true, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, obr, obs, pos);
+        }
         __syncthreads();
     }
 }
@@ -686,7 +747,9 @@ size_t probe_lds_bytes(int kw, int nc) {
 }  // namespace
 
 hipError_t launch_prepare(const PipeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(256), 0, s, a);
+    uint64_t blocks = (a.zero_words + 1023) / 1024;
+    blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);
+    hipLaunchKernelGGL(k_prepare, dim3((unsigned)blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

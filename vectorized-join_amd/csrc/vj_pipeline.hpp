@@ -43,7 +43,10 @@ struct JobDesc {
     uint32_t hist_off;        // u32 offset of the 2^(b1+b2) histogram in the hist arena
     uint32_t base_off;        // u32 offset of the 2^(b1+b2)+1 partition bases in the base arena
     uint32_t p2pre_off;       // u32 offset of the 2^b1+1 pass-2 tile prefix (2-pass jobs)
-    uint32_t hist_blocks;     // histogram blocks of this job
+    uint32_t hist_blocks;     // histogram blocks = pass-1 segments of this job
+    uint32_t seg_tiles;       // pass-1 tiles per segment (segment k = tiles [k*seg_tiles, (k+1)*seg_tiles))
+    uint32_t seg_off;         // offset of this job's segments in the per-segment ticket array
+    uint32_t segc_off;        // offset of this job's (digit, segment) counts / bases, digit-major
     uint32_t p1_tiles;        // pass-1 tiles of this job
     uint32_t p2_tiles_max;    // upper bound of pass-2 tiles
     uint64_t p1_stat_off;     // u64 offset of pass-1 tile status (p1_tiles x 2^b1)
@@ -63,10 +66,8 @@ struct JoinDesc {
 
 // Control words, reset by k_prepare at the start of every pipeline.
 struct Ctl {
-    unsigned int p1_ticket;
-    unsigned int p2_ticket;
     unsigned int p2_total;
-    unsigned int pad;
+    unsigned int pad[3];
     unsigned long long task_ticket;
     unsigned long long task_total;
 };
@@ -81,6 +82,12 @@ struct PipeArgs {
     unsigned long long *join_tasks;  // [n_joins] (device)
     unsigned long long *task0;       // [n_joins + 1] (device)
     unsigned long long *out_count;   // [n_joins] (device)
+    uint32_t *seg_ticket;          // [total segments] pass-1 ticket per segment (device)
+    uint32_t *p2_ticket;           // [p2pre words] pass-2 ticket per pass-1 partition (device)
+    uint32_t *segcnt;              // [total segments x 2^b1] digit counts per segment, digit-major
+    uint32_t *segbase;             // same shape: output base of each (digit, segment)
+    void *zero_begin;              // per-pipeline counters cleared by k_prepare
+    uint64_t zero_words;
     Ctl *ctl;
     uint32_t *hist;
     uint32_t *base;

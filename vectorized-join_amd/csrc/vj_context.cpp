@@ -20,7 +20,6 @@ using namespace vj;
 namespace {
 
 constexpr size_t kAlign = 256;
-constexpr uint32_t kDefaultCapS = 16384;  // probe rows per build+probe task
 constexpr uint32_t kMaxSegments = 256;   // pass-1 look-back sequences per job
 
 inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
@@ -289,7 +288,7 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
         jd.out_s = js.out_s;
         jd.cap = js.cap;
         jd.cap_r = build_cap(pl.kb);
-        jd.cap_s = kDefaultCapS;
+        jd.cap_s = kProbeChunk;
         jd.nparts = 1u << (P >= 1 ? b1 : 0) << (P >= 2 ? b2 : 0);
         jd.ptask_off = (uint32_t)pl.ptask_words;
         pl.ptask_words += jd.nparts + 1;

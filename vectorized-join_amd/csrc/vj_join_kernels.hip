@@ -296,23 +296,24 @@ struct PassTile {
 };
 
 template <int HP, int KW, int NC>
-__global__ __launch_bounds__(kThreads) void k_pass(PipeArgs a, int pass) {
+__global__ __launch_bounds__(kPassThreads) void k_pass(PipeArgs a, int pass) {
     using KT = typename KeyT<KW>::T;
     constexpr int KB = KW * NC;
     constexpr int ITEMS = pass_items(KB);
-    constexpr int T = kThreads * ITEMS;
+    constexpr int T = kPassThreads * ITEMS;
+    constexpr int STAGE = (KB > 4 ? KB : 4) * T;  // key columns, then row ids, staged in turn
     extern __shared__ __align__(16) unsigned char smem[];
-    KT *skeys = reinterpret_cast<KT *>(smem);                         // [NC][T]
-    uint32_t *sids = reinterpret_cast<uint32_t *>(skeys + NC * T);     // [T]
-    uint32_t *wcnt = sids + T;                                         // [kWaves][256]
-    uint32_t *loc = wcnt + kWaves * 256;                               // [256]
-    uint32_t *gdst = loc + 256;                                        // [256]
-    uint8_t *sdig = reinterpret_cast<uint8_t *>(gdst + 256);           // [T]
+    KT *skeys = reinterpret_cast<KT *>(smem);                             // [NC][T]
+    uint32_t *sids = reinterpret_cast<uint32_t *>(smem);                   // [T] (after the keys are out)
+    uint32_t *wcnt = reinterpret_cast<uint32_t *>(smem + STAGE);           // [kPassWaves][256]
+    uint32_t *loc = wcnt + kPassWaves * 256;                               // [256]
+    uint32_t *gdst = loc + 256;                                            // [256]
+    uint8_t *sdig = reinterpret_cast<uint8_t *>(gdst + 256);               // [T]
     __shared__ PassTile ti;
-    __shared__ uint32_t sh_scan[8];
+    __shared__ uint32_t sh_scan[kPassWaves + 1];
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (uint32_t i = tid; i < kWaves * 256; i += kThreads) wcnt[i] = 0;
+    for (uint32_t i = tid; i < kPassWaves * 256; i += kPassThreads) wcnt[i] = 0;
     if (w == 0) {
         // Blocks map statically onto look-back sequences (pass 1: a segment of a
         // job; pass 2: a pass-1 partition of a job), as many blocks as the
@@ -410,7 +411,7 @@ __global__ __launch_bounds__(kThreads) void k_pass(PipeArgs a, int pass) {
     uint32_t cnt = 0;
     if (tid < bins) {
 #pragma unroll
-        for (int ww = 0; ww < kWaves; ww++) {
+        for (int ww = 0; ww < kPassWaves; ww++) {
             uint32_t c = wcnt[ww * 256 + tid];
             wcnt[ww * 256 + tid] = cnt;
             cnt += c;
@@ -443,28 +444,34 @@ __global__ __launch_bounds__(kThreads) void k_pass(PipeArgs a, int pass) {
     }
     __syncthreads();
 
-    // 3. stage rows in LDS ordered by digit
+    // 3. stage the key columns in LDS ordered by digit; consecutive slots of one
+    //    digit go to consecutive output rows
+    uint32_t slot[ITEMS];
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
+        slot[it] = ~0u;
         if (pk[it] != ~0u) {
             const uint32_t d = pk[it] & 255u;
-            const uint32_t slot = loc[d] + wcnt[w * 256 + d] + (pk[it] >> 8);
+            slot[it] = loc[d] + wcnt[w * 256 + d] + (pk[it] >> 8);
 #pragma unroll
-            for (int c = 0; c < NC; c++) skeys[c * T + slot] = keys[it].c[c];
-            sids[slot] = ids[it];
-            sdig[slot] = (uint8_t)d;
+            for (int c = 0; c < NC; c++) skeys[c * T + slot[it]] = keys[it].c[c];
+            sdig[slot[it]] = (uint8_t)d;
         }
     }
     __syncthreads();
-
-    // 4. write runs: consecutive slots of one digit go to consecutive rows
     const uint32_t n = hi - lo;
-    for (uint32_t k = tid; k < n; k += kThreads) {
+    for (uint32_t k = tid; k < n; k += kPassThreads) {
         const uint32_t dst = gdst[sdig[k]] + k;
 #pragma unroll
         for (int c = 0; c < NC; c++) static_cast<KT *>(ti.dst_cols[c])[dst] = skeys[c * T + k];
-        ti.dst_ids[dst] = sids[k];
     }
+    __syncthreads();
+    // 4. the row ids, through the same staging buffer
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++)
+        if (slot[it] != ~0u) sids[slot[it]] = ids[it];
+    __syncthreads();
+    for (uint32_t k = tid; k < n; k += kPassThreads) ti.dst_ids[gdst[sdig[k]] + k] = sids[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -481,10 +488,11 @@ struct ProbeTask {
     uint32_t nslots, shift, one_based;
     uint32_t dup;
     uint32_t done;
-    uint32_t wave_count[kWaves];
+    uint32_t wave_count[kProbeWaves];
 };
 
 constexpr int kProbeUnroll = 8;
+constexpr uint16_t kNoMatch = 0xFFFF;
 
 // Writes the wave's staged pairs to rows [pos, pos + wn) of the join output.
 __device__ __forceinline__ void flush_wave(const uint32_t *obr, const uint32_t *obs, uint32_t wn,
@@ -502,91 +510,140 @@ __device__ __forceinline__ void flush_wave(const uint32_t *obr, const uint32_t *
     __builtin_amdgcn_wave_barrier();
 }
 
-// Probes this wave's rows of the S chunk against the LDS table. EMIT = false
-// only counts matches; EMIT = true stages them in the wave's LDS buffer and
-// writes them from row `pos` on. Both walks visit rows and table slots in the
-// same order, so the counts of the first walk are the row counts of the second.
-template <bool EMIT, int HP, int KW, int NC>
+// Per-wave staging of output pairs; rows go out in buffer-sized runs.
+struct Emitter {
+    uint32_t *obr, *obs;
+    uint32_t wn;
+    unsigned long long pos;
+    __device__ __forceinline__ void put(bool m, uint32_t r, uint32_t s, const ProbeTask &t) {
+        const uint64_t mb = __ballot(m);
+        if (!mb) return;
+        const uint32_t c = __builtin_popcountll(mb);
+        if (wn + c > (uint32_t)kOutBuf) {
+            flush_wave(obr, obs, wn, pos, t);
+            pos += wn;
+            wn = 0;
+        }
+        if (m) {
+            const uint32_t p = wn + lanes_below(mb);
+            obr[p] = r;
+            obs[p] = s;
+        }
+        wn += c;
+    }
+    __device__ __forceinline__ void finish(const ProbeTask &t) {
+        if (wn) flush_wave(obr, obs, wn, pos, t);
+        pos += wn;
+        wn = 0;
+    }
+};
+
+// Walk modes over this wave's rows of the S chunk (rows base + u*64 + lane):
+//   WALK_COUNT   probe, count matches                      (build chunk has duplicate keys)
+//   WALK_EMIT    probe again, write the matches            (duplicates)
+//   WALK_RECORD  probe, count, record each row's match in LDS (no duplicates: <= 1 match per row)
+//   WALK_REPLAY  write the recorded matches, no probing
+// All walks visit rows in the same order, so the counts of the first walk
+// are the row counts of the second. Probing runs in rounds: each round reads
+// one table slot for every unfinished row of the lane, so kProbeUnroll
+// independent LDS chains are in flight per lane.
+enum { WALK_COUNT = 0, WALK_EMIT = 1, WALK_RECORD = 2, WALK_REPLAY = 3 };
+
+template <int MODE, int HP, int KW, int NC>
 __device__ __forceinline__ uint32_t probe_walk(const ProbeTask &t, const HashCfg &hc, uint32_t tag_shift,
                                                const typename KeyT<KW>::T *rk, const uint32_t *rid,
-                                               const uint32_t *slots, uint32_t *obr, uint32_t *obs,
-                                               unsigned long long pos) {
+                                               const uint32_t *slots, uint16_t *res, Emitter &em) {
     constexpr int CAP = build_cap(KW * NC);
+    constexpr int U = kProbeUnroll;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t smask = t.nslots - 1, shift = t.shift, ob1 = t.one_based;
     const bool dup = t.dup != 0;
-    const uint32_t s_end = t.s_lo + t.s_n;
-    uint32_t total = 0, wn = 0;
-    for (uint32_t base = t.s_lo + w * 64 * kProbeUnroll; base < s_end; base += kThreads * kProbeUnroll) {
-        Key<KW, NC> sk[kProbeUnroll];
-        uint32_t sid[kProbeUnroll];
+    const uint32_t s_lo = t.s_lo, s_end = t.s_lo + t.s_n;
+    uint32_t total = 0;
+    for (uint32_t base = s_lo + w * 64 * U; base < s_end; base += kProbeThreads * U) {
+        if (MODE == WALK_REPLAY) {
 #pragma unroll
-        for (int u = 0; u < kProbeUnroll; u++) {
+            for (int u = 0; u < U; u++) {
+                const uint32_t i = base + u * 64 + lane;
+                bool m = false;
+                uint32_t rr = 0, ss = 0;
+                if (i < s_end) {
+                    const uint32_t o = res[i - s_lo];
+                    if (o != kNoMatch) {
+                        m = true;
+                        rr = rid[o] + ob1;
+                        ss = (t.s.rowids ? t.s.rowids[i] : i) + ob1;
+                    }
+                }
+                em.put(m, rr, ss, t);
+            }
+            continue;
+        }
+        Key<KW, NC> sk[U];
+        uint32_t sid[U], sl[U], tag[U];
+        uint32_t act = 0;  // bit u: row u of this lane still probing
+#pragma unroll
+        for (int u = 0; u < U; u++) {
             const uint32_t i = base + u * 64 + lane;
             if (i < s_end) {
                 sk[u] = load_key<KW, NC>(t.s, i);
-                if (EMIT) sid[u] = t.s.rowids ? t.s.rowids[i] : i;
+                if (MODE == WALK_EMIT) sid[u] = t.s.rowids ? t.s.rowids[i] : i;
+                act |= 1u << u;
             }
         }
 #pragma unroll
-        for (int u = 0; u < kProbeUnroll; u++) {
-            const uint32_t i = base + u * 64 + lane;
-            bool active = i < s_end;
-            uint32_t sl = 0, tag = 0;
-            if (active) {
+        for (int u = 0; u < U; u++) {
+            sl[u] = 0;
+            tag[u] = 0;
+            if (act & (1u << u)) {
                 const uint64_t h = hash_key<HP, KW, NC>(sk[u], hc);
-                tag = (uint32_t)(h >> tag_shift) & 0xFFFFu;
-                sl = (uint32_t)(h >> shift) & smask;
+                tag[u] = (uint32_t)(h >> tag_shift) & 0xFFFFu;
+                sl[u] = (uint32_t)(h >> shift) & smask;
             }
-            while (__ballot(active)) {
+            if (MODE == WALK_RECORD) {
+                const uint32_t i = base + u * 64 + lane;
+                if (i < s_end) res[i - s_lo] = kNoMatch;
+            }
+        }
+        while (__ballot(act != 0)) {
+            uint32_t e[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) e[u] = (act & (1u << u)) ? slots[sl[u]] : 0u;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
                 bool m = false;
-                uint32_t rrow = 0;
-                if (active) {
-                    const uint32_t e = slots[sl];
-                    if (e == 0) {
-                        active = false;
+                uint32_t o = 0;
+                if (act & (1u << u)) {
+                    if (e[u] == 0) {
+                        act &= ~(1u << u);
                     } else {
-                        if ((e >> 16) == tag) {
-                            const uint32_t o = (e & 0xFFFFu) - 1;
+                        if ((e[u] >> 16) == tag[u]) {
+                            o = (e[u] & 0xFFFFu) - 1;
                             bool eq = true;
 #pragma unroll
                             for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == sk[u].c[c];
                             if (eq) {
                                 m = true;
-                                if (EMIT) rrow = rid[o];
-                                if (!dup) active = false;
+                                if (!dup) act &= ~(1u << u);
                             }
                         }
-                        sl = (sl + 1) & smask;
+                        sl[u] = (sl[u] + 1) & smask;
                     }
                 }
-                const uint64_t mb = __ballot(m);
-                if (mb) {
-                    const uint32_t c = __builtin_popcountll(mb);
-                    if (EMIT) {
-                        if (wn + c > (uint32_t)kOutBuf) {
-                            flush_wave(obr, obs, wn, pos, t);
-                            pos += wn;
-                            wn = 0;
-                        }
-                        if (m) {
-                            const uint32_t p = wn + lanes_below(mb);
-                            obr[p] = rrow + ob1;
-                            obs[p] = sid[u] + ob1;
-                        }
-                        wn += c;
-                    }
-                    total += c;
+                if (MODE == WALK_EMIT) {
+                    em.put(m, rid[o] + ob1, sid[u] + ob1, t);
+                } else {
+                    if (MODE == WALK_RECORD && m) res[base + u * 64 + lane - s_lo] = (uint16_t)o;
+                    total += __builtin_popcountll(__ballot(m));
                 }
             }
         }
     }
-    if (EMIT && wn) flush_wave(obr, obs, wn, pos, t);
     return total;
 }
 
 template <int HP, int KW, int NC>
-__global__ __launch_bounds__(kThreads) void k_probe(PipeArgs a) {
+__global__ __launch_bounds__(kProbeThreads) void k_probe(PipeArgs a) {
     using KT = typename KeyT<KW>::T;
     constexpr int KB = KW * NC;
     constexpr int CAP = build_cap(KB);
@@ -594,11 +651,14 @@ __global__ __launch_bounds__(kThreads) void k_probe(PipeArgs a) {
     KT *rk = reinterpret_cast<KT *>(smem);                       // [NC][CAP]
     uint32_t *rid = reinterpret_cast<uint32_t *>(rk + NC * CAP);  // [CAP]
     uint32_t *slots = rid + CAP;                                  // [2 * CAP]
-    uint32_t *ob = slots + 2 * CAP;                               // [kWaves][2][kOutBuf]
+    uint32_t *ob = slots + 2 * CAP;                               // [kProbeWaves][2][kOutBuf]
+    uint16_t *res = reinterpret_cast<uint16_t *>(ob + kProbeWaves * 2 * kOutBuf);  // [kProbeChunk]
     __shared__ ProbeTask t;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    uint32_t *obr = ob + w * 2 * kOutBuf, *obs = obr + kOutBuf;
+    Emitter em;
+    em.obr = ob + w * 2 * kOutBuf;
+    em.obs = em.obr + kOutBuf;
     const HashCfg hc = a.hash;
     const unsigned long long total = a.ctl->task_total;
     const uint32_t tag_shift = hc.bits - 16;
@@ -659,15 +719,15 @@ __global__ __launch_bounds__(kThreads) void k_probe(PipeArgs a) {
 
         // build: load the R chunk, then insert (index + 1, 16-bit tag) with linear probing
         const uint32_t r_lo = t.r_lo, r_n = t.r_n, nslots = t.nslots, smask = nslots - 1, shift = t.shift;
-        for (uint32_t i = tid; i < nslots; i += kThreads) slots[i] = 0;
-        for (uint32_t i = tid; i < r_n; i += kThreads) {
+        for (uint32_t i = tid; i < nslots; i += kProbeThreads) slots[i] = 0;
+        for (uint32_t i = tid; i < r_n; i += kProbeThreads) {
             Key<KW, NC> k = load_key<KW, NC>(t.r, r_lo + i);
 #pragma unroll
             for (int c = 0; c < NC; c++) rk[c * CAP + i] = k.c[c];
             rid[i] = t.r.rowids ? t.r.rowids[r_lo + i] : r_lo + i;
         }
         __syncthreads();
-        for (uint32_t i = tid; i < r_n; i += kThreads) {
+        for (uint32_t i = tid; i < r_n; i += kProbeThreads) {
             Key<KW, NC> k;
 #pragma unroll
             for (int c = 0; c < NC; c++) k.c[c] = rk[c * CAP + i];
@@ -690,20 +750,26 @@ __global__ __launch_bounds__(kThreads) void k_probe(PipeArgs a) {
         }
         __syncthreads();
 
-        // count, reserve the task's output rows with one atomic, then write
-        const uint32_t wc = probe_walk<false, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, obr, obs, 0);
+        // count (recording matches when the chunk has no duplicate keys), reserve
+        // the task's output rows with one atomic, then write them
+        const bool dup = t.dup != 0;
+        const uint32_t wc = dup ? probe_walk<WALK_COUNT, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, res, em)
+                                : probe_walk<WALK_RECORD, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, res, em);
         if (lane == 0) t.wave_count[w] = wc;
         __syncthreads();
         if (tid == 0) {
             uint32_t sum = 0;
-            for (int k = 0; k < kWaves; k++) sum += t.wave_count[k];
+            for (int k = 0; k < kProbeWaves; k++) sum += t.wave_count[k];
             t.out_base = sum ? atomicAdd(t.count, (unsigned long long)sum) : 0;
         }
         __syncthreads();
         if (wc) {
-            unsigned long long pos = t.out_base;
-            for (uint32_t k = 0; k < w; k++) pos += t.wave_count[k];
-            probe_walk<true, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, obr, obs, pos);
+            em.pos = t.out_base;
+            em.wn = 0;
+            for (uint32_t k = 0; k < w; k++) em.pos += t.wave_count[k];
+            if (dup) probe_walk<WALK_EMIT, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, res, em);
+            else probe_walk<WALK_REPLAY, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, res, em);
+            em.finish(t);
         }
         __syncthreads();
     }
@@ -736,12 +802,13 @@ template <typename F> hipError_t dispatch(const LaunchShape &sh, F &&f) {
 
 size_t pass_lds_bytes(int kw, int nc) {
     const int kb = kw * nc, T = pass_tile(kb);
-    return (size_t)T * kb + (size_t)T * 4 + (size_t)(kWaves * 256 + 512) * 4 + (size_t)T;
+    return (size_t)T * (kb > 4 ? kb : 4) + (size_t)(kPassWaves * 256 + 512) * 4 + (size_t)T;
 }
 
 size_t probe_lds_bytes(int kw, int nc) {
     const int kb = kw * nc, cap = build_cap(kb);
-    return (size_t)cap * kb + (size_t)cap * 4 + (size_t)cap * 8 + (size_t)kWaves * 2 * kOutBuf * 4;
+    return (size_t)cap * kb + (size_t)cap * 4 + (size_t)cap * 8 + (size_t)kProbeWaves * 2 * kOutBuf * 4 +
+           (size_t)kProbeChunk * 2;
 }
 
 }  // namespace
@@ -785,7 +852,7 @@ hipError_t launch_pass(const PipeArgs &a, const LaunchShape &sh, int pass, hipSt
     if (grid == 0) return hipSuccess;
     const size_t lds = pass_lds_bytes(sh.kw, sh.nc);
     return dispatch(sh, [&]<int HP, int KW, int NC>() {
-        hipLaunchKernelGGL((k_pass<HP, KW, NC>), dim3(grid), dim3(kThreads), lds, s, a, pass);
+        hipLaunchKernelGGL((k_pass<HP, KW, NC>), dim3(grid), dim3(kPassThreads), lds, s, a, pass);
         return hipGetLastError();
     });
 }
@@ -799,7 +866,7 @@ int probe_blocks_per_cu(const LaunchShape &sh) {
 hipError_t launch_probe(const PipeArgs &a, const LaunchShape &sh, int blocks, hipStream_t s) {
     const size_t lds = probe_lds_bytes(sh.kw, sh.nc);
     return dispatch(sh, [&]<int HP, int KW, int NC>() {
-        hipLaunchKernelGGL((k_probe<HP, KW, NC>), dim3(blocks), dim3(kThreads), lds, s, a);
+        hipLaunchKernelGGL((k_probe<HP, KW, NC>), dim3(blocks), dim3(kProbeThreads), lds, s, a);
         return hipGetLastError();
     });
 }

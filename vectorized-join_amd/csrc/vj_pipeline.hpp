@@ -16,16 +16,19 @@
 
 namespace vj {
 
-constexpr int kThreads = 256;  // every pipeline kernel
-constexpr int kWaves = kThreads / 64;
+constexpr int kPassThreads = 512;   // radix pass kernel
+constexpr int kPassWaves = kPassThreads / 64;
+constexpr int kProbeThreads = 512;  // build+probe kernel
+constexpr int kProbeWaves = kProbeThreads / 64;
 constexpr int kMaxPassBits = 8;   // digits per pass <= 256
 constexpr int kMaxHistBits = 14;  // 2D histogram in LDS: 2^14 x 4 B = 64 KiB
-constexpr int kOutBuf = 256;      // per-wave LDS output staging (pairs)
+constexpr int kOutBuf = 128;      // per-wave LDS output staging (pairs)
 
-// Tuples per pass tile and per build chunk, by key bytes per row.
-__host__ __device__ constexpr int pass_items(int key_bytes) { return key_bytes <= 8 ? 16 : (key_bytes <= 16 ? 8 : 4); }
-__host__ __device__ constexpr int pass_tile(int key_bytes) { return kThreads * pass_items(key_bytes); }
+// Tuples per pass tile, per build chunk and per probe chunk, by key bytes per row.
+__host__ __device__ constexpr int pass_items(int key_bytes) { return key_bytes <= 8 ? 8 : (key_bytes <= 16 ? 4 : 2); }
+__host__ __device__ constexpr int pass_tile(int key_bytes) { return kPassThreads * pass_items(key_bytes); }
 __host__ __device__ constexpr int build_cap(int key_bytes) { return key_bytes <= 8 ? 2048 : 1024; }
+constexpr int kProbeChunk = 8192;  // probe rows per build+probe task (LDS match slots)
 
 // Status word of the decoupled look-back: flag in bits 62..63, count in 0..31.
 constexpr unsigned long long kFlagAgg = 1ull << 62;

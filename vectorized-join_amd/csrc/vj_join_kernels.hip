@@ -491,7 +491,7 @@ struct ProbeTask {
     uint32_t wave_count[kProbeWaves];
 };
 
-constexpr int kProbeUnroll = 8;
+constexpr int kProbeUnroll = 4;
 constexpr uint16_t kNoMatch = 0xFFFF;
 
 // Writes the wave's staged pairs to rows [pos, pos + wn) of the join output.
@@ -538,21 +538,34 @@ struct Emitter {
     }
 };
 
-// Walk modes over this wave's rows of the S chunk (rows base + u*64 + lane):
-//   WALK_COUNT   probe, count matches                      (build chunk has duplicate keys)
-//   WALK_EMIT    probe again, write the matches            (duplicates)
-//   WALK_RECORD  probe, count, record each row's match in LDS (no duplicates: <= 1 match per row)
-//   WALK_REPLAY  write the recorded matches, no probing
+// Walks over this wave's rows of the S chunk. The chunk holds up to
+// kProbeChunk = kProbeIters x kProbeThreads x kProbeUnroll rows; in iteration
+// `it` lane `l` of wave `w` owns rows it*kProbeThreads*U + w*64*U + u*64 + l.
+//   WALK_COUNT   probe, count matches                          (build chunk has duplicate keys)
+//   WALK_EMIT    probe again, write the matches                (duplicates)
+//   WALK_RECORD  probe, count, keep each row's match in registers (no duplicates: <= 1 match)
+//   WALK_REPLAY  write the kept matches, no probing
 // All walks visit rows in the same order, so the counts of the first walk
 // are the row counts of the second. Probing runs in rounds: each round reads
 // one table slot for every unfinished row of the lane, so kProbeUnroll
 // independent LDS chains are in flight per lane.
 enum { WALK_COUNT = 0, WALK_EMIT = 1, WALK_RECORD = 2, WALK_REPLAY = 3 };
 
+template <int N> __device__ __forceinline__ uint32_t res_get(const uint32_t (&r)[N], int k) {
+    return (r[k >> 1] >> ((k & 1) * 16)) & 0xFFFFu;
+}
+template <int N> __device__ __forceinline__ void res_set(uint32_t (&r)[N], int k, uint32_t v) {
+    const int sh = (k & 1) * 16;
+    r[k >> 1] = (r[k >> 1] & ~(0xFFFFu << sh)) | (v << sh);
+}
+constexpr int kProbeIters = kProbeChunk / (kProbeThreads * kProbeUnroll);
+static_assert(kProbeIters * kProbeThreads * kProbeUnroll == kProbeChunk, "probe chunk shape");
+
 template <int MODE, int HP, int KW, int NC>
 __device__ __forceinline__ uint32_t probe_walk(const ProbeTask &t, const HashCfg &hc, uint32_t tag_shift,
                                                const typename KeyT<KW>::T *rk, const uint32_t *rid,
-                                               const uint32_t *slots, uint16_t *res, Emitter &em) {
+                                               const uint32_t *slots, uint32_t (&res)[kProbeIters * kProbeUnroll / 2],
+                                               Emitter &em) {
     constexpr int CAP = build_cap(KW * NC);
     constexpr int U = kProbeUnroll;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
@@ -560,22 +573,23 @@ __device__ __forceinline__ uint32_t probe_walk(const ProbeTask &t, const HashCfg
     const bool dup = t.dup != 0;
     const uint32_t s_lo = t.s_lo, s_end = t.s_lo + t.s_n;
     uint32_t total = 0;
-    for (uint32_t base = s_lo + w * 64 * U; base < s_end; base += kProbeThreads * U) {
+#pragma unroll
+    for (int it = 0; it < kProbeIters; it++) {
+        const uint32_t base = s_lo + it * kProbeThreads * U + w * 64 * U;
+        if (base >= s_end) break;  // wave-uniform
         if (MODE == WALK_REPLAY) {
+            uint32_t sid[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const uint32_t i = base + u * 64 + lane;
-                bool m = false;
-                uint32_t rr = 0, ss = 0;
-                if (i < s_end) {
-                    const uint32_t o = res[i - s_lo];
-                    if (o != kNoMatch) {
-                        m = true;
-                        rr = rid[o] + ob1;
-                        ss = (t.s.rowids ? t.s.rowids[i] : i) + ob1;
-                    }
-                }
-                em.put(m, rr, ss, t);
+                sid[u] = 0;
+                if (i < s_end && res_get(res, it * U + u) != kNoMatch) sid[u] = t.s.rowids ? t.s.rowids[i] : i;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint32_t o = res_get(res, it * U + u);
+                const bool m = o != kNoMatch;
+                em.put(m, (m ? rid[o] : 0u) + ob1, sid[u] + ob1, t);
             }
             continue;
         }
@@ -585,6 +599,7 @@ __device__ __forceinline__ uint32_t probe_walk(const ProbeTask &t, const HashCfg
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const uint32_t i = base + u * 64 + lane;
+            if (MODE == WALK_RECORD) res_set(res, it * U + u, kNoMatch);
             if (i < s_end) {
                 sk[u] = load_key<KW, NC>(t.s, i);
                 if (MODE == WALK_EMIT) sid[u] = t.s.rowids ? t.s.rowids[i] : i;
@@ -599,10 +614,6 @@ __device__ __forceinline__ uint32_t probe_walk(const ProbeTask &t, const HashCfg
                 const uint64_t h = hash_key<HP, KW, NC>(sk[u], hc);
                 tag[u] = (uint32_t)(h >> tag_shift) & 0xFFFFu;
                 sl[u] = (uint32_t)(h >> shift) & smask;
-            }
-            if (MODE == WALK_RECORD) {
-                const uint32_t i = base + u * 64 + lane;
-                if (i < s_end) res[i - s_lo] = kNoMatch;
             }
         }
         while (__ballot(act != 0)) {
@@ -633,7 +644,7 @@ __device__ __forceinline__ uint32_t probe_walk(const ProbeTask &t, const HashCfg
                 if (MODE == WALK_EMIT) {
                     em.put(m, rid[o] + ob1, sid[u] + ob1, t);
                 } else {
-                    if (MODE == WALK_RECORD && m) res[base + u * 64 + lane - s_lo] = (uint16_t)o;
+                    if (MODE == WALK_RECORD && m) res_set(res, it * U + u, o);
                     total += __builtin_popcountll(__ballot(m));
                 }
             }
@@ -643,7 +654,7 @@ __device__ __forceinline__ uint32_t probe_walk(const ProbeTask &t, const HashCfg
 }
 
 template <int HP, int KW, int NC>
-__global__ __launch_bounds__(kProbeThreads) void k_probe(PipeArgs a) {
+__global__ __launch_bounds__(kProbeThreads, 6) void k_probe(PipeArgs a) {
     using KT = typename KeyT<KW>::T;
     constexpr int KB = KW * NC;
     constexpr int CAP = build_cap(KB);
@@ -652,8 +663,8 @@ __global__ __launch_bounds__(kProbeThreads) void k_probe(PipeArgs a) {
     uint32_t *rid = reinterpret_cast<uint32_t *>(rk + NC * CAP);  // [CAP]
     uint32_t *slots = rid + CAP;                                  // [2 * CAP]
     uint32_t *ob = slots + 2 * CAP;                               // [kProbeWaves][2][kOutBuf]
-    uint16_t *res = reinterpret_cast<uint16_t *>(ob + kProbeWaves * 2 * kOutBuf);  // [kProbeChunk]
     __shared__ ProbeTask t;
+    uint32_t res[kProbeIters * kProbeUnroll / 2];  // matched build row per probe row, two per word
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     Emitter em;
@@ -807,8 +818,7 @@ size_t pass_lds_bytes(int kw, int nc) {
 
 size_t probe_lds_bytes(int kw, int nc) {
     const int kb = kw * nc, cap = build_cap(kb);
-    return (size_t)cap * kb + (size_t)cap * 4 + (size_t)cap * 8 + (size_t)kProbeWaves * 2 * kOutBuf * 4 +
-           (size_t)kProbeChunk * 2;
+    return (size_t)cap * kb + (size_t)cap * 4 + (size_t)cap * 8 + (size_t)kProbeWaves * 2 * kOutBuf * 4;
 }
 
 }  // namespace

@@ -28,7 +28,7 @@ constexpr int kOutBuf = 128;      // per-wave LDS output staging (pairs)
 __host__ __device__ constexpr int pass_items(int key_bytes) { return key_bytes <= 8 ? 8 : (key_bytes <= 16 ? 4 : 2); }
 __host__ __device__ constexpr int pass_tile(int key_bytes) { return kPassThreads * pass_items(key_bytes); }
 __host__ __device__ constexpr int build_cap(int key_bytes) { return key_bytes <= 8 ? 2048 : 1024; }
-constexpr int kProbeChunk = 8192;  // probe rows per build+probe task (LDS match slots)
+constexpr int kProbeChunk = 16384;  // probe rows per build+probe task
 
 // Status word of the decoupled look-back: flag in bits 62..63, count in 0..31.
 constexpr unsigned long long kFlagAgg = 1ull << 62;

@@ -20,7 +20,6 @@ using namespace vj;
 namespace {
 
 constexpr size_t kAlign = 256;
-constexpr uint32_t kMaxSegments = 256;   // pass-1 look-back sequences per job
 
 inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
@@ -40,8 +39,7 @@ struct vj_ctx {
     bool profiling = false;
     std::string last_error;
     vj_join_summary summary{};
-    DevBuf tuples, hist, base, p2pre, ptask, status, segs, meta;
-    bool hist_dirty = true;
+    DevBuf tuples, counts, base, ptask, meta;
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
     hipEvent_t ev[EV_COUNT] = {};
@@ -52,8 +50,7 @@ namespace {
 // --- memory ----------------------------------------------------------------
 
 uint64_t held_bytes(const vj_ctx *c) {
-    return c->tuples.bytes + c->hist.bytes + c->base.bytes + c->p2pre.bytes + c->ptask.bytes + c->status.bytes +
-           c->segs.bytes + c->meta.bytes;
+    return c->tuples.bytes + c->counts.bytes + c->base.bytes + c->ptask.bytes + c->meta.bytes;
 }
 
 void free_buf(DevBuf &b) {
@@ -64,14 +61,10 @@ void free_buf(DevBuf &b) {
 
 void free_all(vj_ctx *c) {
     free_buf(c->tuples);
-    free_buf(c->hist);
+    free_buf(c->counts);
     free_buf(c->base);
-    free_buf(c->p2pre);
     free_buf(c->ptask);
-    free_buf(c->status);
-    free_buf(c->segs);
     free_buf(c->meta);
-    c->hist_dirty = true;
 }
 
 Err ensure(DevBuf &b, size_t bytes) {
@@ -109,20 +102,19 @@ struct JoinSpec {
 struct Plan {
     std::vector<JobDesc> jobs;
     std::vector<JoinDesc> joins;
-    std::vector<uint32_t> hist_block0, p1_tile0;
+    std::vector<ColScanDesc> colscan;
+    std::vector<uint32_t> hist_block0, p1_tile0, p2_chunk0;
     std::vector<size_t> fin_off, mid_off;  // byte offsets in the tuples arena (SIZE_MAX: none)
     size_t tuples_bytes = 0;
-    uint64_t hist_words = 0, base_words = 0, p2pre_words = 0, ptask_words = 0, status_words = 0;
-    uint64_t seg_words = 0, segc_words = 0;  // per-segment tickets; (digit, segment) counts
-    uint32_t max_hist_bins = 1, total_hist_blocks = 0, total_p1_tiles = 0, total_p2_tiles_max = 0;
+    uint64_t counts_words = 0, base_words = 0, ptask_words = 0;
+    uint32_t max_hist_bins = 1, total_hist_blocks = 0, total_p1_tiles = 0, total_p2_chunks = 0;
     bool any_part = false, any_p2 = false;
     LaunchShape shape{};
     HashCfg hash{};
     int kb = 0;
     // meta layout
-    size_t off_jobs = 0, off_joins = 0, off_hb0 = 0, off_pt0 = 0, upload_bytes = 0;
-    size_t off_jp2 = 0, off_p2t0 = 0, off_jtasks = 0, off_task0 = 0, off_count = 0, off_ctl = 0, meta_bytes = 0;
-    size_t off_segt = 0, off_p2t = 0, zero_begin = 0, zero_end = 0;
+    size_t off_jobs = 0, off_joins = 0, off_cs = 0, off_hb0 = 0, off_pt0 = 0, off_pc0 = 0, upload_bytes = 0;
+    size_t off_jtasks = 0, off_task0 = 0, off_count = 0, off_ctl = 0, meta_bytes = 0;
 };
 
 int ceil_log2(uint64_t v) {
@@ -168,6 +160,8 @@ Err check_pair(const vj_table *r, const vj_table *s) {
     return Err::ok();
 }
 
+constexpr uint32_t kMaxSegments = 256;  // segments (histogram blocks) per job
+
 void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0) {
     JobDesc jd;
     memset(&jd, 0, sizeof(jd));
@@ -177,44 +171,45 @@ void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0
     jd.b2 = P >= 2 ? b2 : 0;
     jd.shift0 = shift0;
     const uint32_t T = pass_tile(pl.kb);
-    const uint32_t nb = 1u << (jd.b1 + jd.b2);
+    const uint32_t n1 = 1u << jd.b1, nb = 1u << (jd.b1 + jd.b2);
     jd.base_off = (uint32_t)pl.base_words;
     pl.base_words += (jd.b1 ? nb : 1) + 1;
-    size_t row_bytes_cols = 0;
-    if (jd.b1) {
+    const uint32_t job_index = (uint32_t)pl.jobs.size();
+    uint32_t chunks = 0;
+    if (jd.b1 && jd.n) {
         pl.any_part = true;
-        jd.hist_off = (uint32_t)pl.hist_words;
-        pl.hist_words += nb;
-        pl.max_hist_bins = std::max(pl.max_hist_bins, nb);
         jd.p1_tiles = (uint32_t)(((uint64_t)jd.n + T - 1) / T);
-        // pass-1 segments (= histogram blocks): at most kMaxSegments per job
-        jd.seg_tiles = jd.p1_tiles ? (jd.p1_tiles + kMaxSegments - 1) / kMaxSegments : 1;
-        jd.hist_blocks = jd.p1_tiles ? (jd.p1_tiles + jd.seg_tiles - 1) / jd.seg_tiles : 0;
-        jd.seg_off = (uint32_t)pl.seg_words;
-        pl.seg_words += jd.hist_blocks;
-        jd.segc_off = (uint32_t)pl.segc_words;
-        pl.segc_words += (uint64_t)jd.hist_blocks << jd.b1;
-        jd.p1_stat_off = pl.status_words;
-        pl.status_words += (uint64_t)jd.p1_tiles << jd.b1;
+        // segments: pass-2 chunks (a segment's rows of one pass-1 partition)
+        // of about one tile; for one pass, enough segments to spread the count
+        uint64_t want = jd.b2 ? ((uint64_t)jd.n + (uint64_t)n1 * T - 1) / ((uint64_t)n1 * T)
+                              : ((uint64_t)jd.p1_tiles + 15) / 16;
+        want = std::max<uint64_t>(1, std::min<uint64_t>({want, kMaxSegments, jd.p1_tiles}));
+        jd.seg_tiles = (uint32_t)((jd.p1_tiles + want - 1) / want);
+        jd.nseg = (jd.p1_tiles + jd.seg_tiles - 1) / jd.seg_tiles;
+        jd.tilepre_off = pl.counts_words;
+        pl.counts_words += (uint64_t)jd.p1_tiles * n1;
+        jd.m1_off = pl.counts_words;
+        pl.counts_words += (uint64_t)(jd.nseg + 1) * n1;
+        for (uint32_t c0 = 0; c0 < n1; c0 += 256) pl.colscan.push_back({job_index, 1u, c0, 0u});
         if (jd.b2) {
             pl.any_p2 = true;
-            jd.p2pre_off = (uint32_t)pl.p2pre_words;
-            pl.p2pre_words += (1u << jd.b1) + 1;
-            jd.p2_tiles_max = jd.n ? (uint32_t)(((uint64_t)jd.n + T - 1) / T) + (1u << jd.b1) : 0;
-            jd.p2_stat_off = pl.status_words;
-            pl.status_words += (uint64_t)jd.p2_tiles_max << jd.b2;
+            jd.m2_off = pl.counts_words;
+            pl.counts_words += (uint64_t)(jd.nseg + 1) * nb;
+            for (uint32_t c0 = 0; c0 < nb; c0 += 256) pl.colscan.push_back({job_index, 2u, c0, 0u});
+            chunks = n1 * jd.nseg;
         }
-        row_bytes_cols = (size_t)t->key_width * jd.n;
+        pl.max_hist_bins = std::max(pl.max_hist_bins, (jd.b2 ? nb : 0) + n1 * (1 + kHistThreads / 64));
     }
     pl.hist_block0.push_back(pl.total_hist_blocks);
-    pl.total_hist_blocks += jd.hist_blocks;
+    pl.total_hist_blocks += jd.nseg;
     pl.p1_tile0.push_back(pl.total_p1_tiles);
     pl.total_p1_tiles += jd.p1_tiles;
-    pl.total_p2_tiles_max += jd.p2_tiles_max;
+    pl.p2_chunk0.push_back(pl.total_p2_chunks);
+    pl.total_p2_chunks += chunks;
     // tuple buffers: fin (and mid for two passes), SoA key columns + row ids
     size_t fin = SIZE_MAX, mid = SIZE_MAX;
     if (jd.b1) {
-        const size_t one = align_up(row_bytes_cols) * t->n_cols + align_up((size_t)4 * jd.n);
+        const size_t one = align_up((size_t)t->key_width * jd.n) * t->n_cols + align_up((size_t)4 * jd.n);
         fin = pl.tuples_bytes;
         pl.tuples_bytes += one;
         if (jd.b2) {
@@ -229,38 +224,32 @@ void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0
 
 void finish_layout(Plan &pl) {
     const size_t nj = pl.jobs.size(), nn = pl.joins.size();
+    pl.hist_block0.push_back(pl.total_hist_blocks);
+    pl.p1_tile0.push_back(pl.total_p1_tiles);
+    pl.p2_chunk0.push_back(pl.total_p2_chunks);
     size_t o = 0;
     pl.off_jobs = o;
     o = align_up(o + nj * sizeof(JobDesc));
     pl.off_joins = o;
     o = align_up(o + nn * sizeof(JoinDesc));
+    pl.off_cs = o;
+    o = align_up(o + pl.colscan.size() * sizeof(ColScanDesc));
     pl.off_hb0 = o;
     o = align_up(o + (nj + 1) * 4);
     pl.off_pt0 = o;
     o = align_up(o + (nj + 1) * 4);
+    pl.off_pc0 = o;
+    o = align_up(o + (nj + 1) * 4);
     pl.upload_bytes = o;
-    // counters cleared by k_prepare: one contiguous region
-    pl.zero_begin = o;
-    pl.off_jp2 = o;
-    o = align_up(o + nj * 4);
     pl.off_jtasks = o;
     o = align_up(o + nn * 8);
-    pl.off_count = o;
-    o = align_up(o + nn * 8);
-    pl.off_segt = o;
-    o = align_up(o + pl.seg_words * 4);
-    pl.off_p2t = o;
-    o = align_up(o + pl.p2pre_words * 4);
-    pl.zero_end = o;
-    pl.off_p2t0 = o;
-    o = align_up(o + (nj + 1) * 4);
     pl.off_task0 = o;
     o = align_up(o + (nn + 1) * 8);
+    pl.off_count = o;
+    o = align_up(o + nn * 8);
     pl.off_ctl = o;
     o = align_up(o + sizeof(Ctl));
     pl.meta_bytes = o;
-    pl.hist_block0.push_back(pl.total_hist_blocks);
-    pl.p1_tile0.push_back(pl.total_p1_tiles);
 }
 
 Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_opts *o) {
@@ -301,9 +290,8 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
 }
 
 uint64_t plan_scratch_bytes(const Plan &pl) {
-    return align_up(pl.tuples_bytes) + align_up(pl.hist_words * 4) + align_up(pl.base_words * 4) +
-           align_up(pl.p2pre_words * 4) + align_up(pl.ptask_words * 8) + align_up(pl.status_words * 8) +
-           align_up(pl.segc_words * 8) + align_up(pl.meta_bytes);
+    return align_up(pl.tuples_bytes) + align_up(pl.counts_words * 4) + align_up(pl.base_words * 4) +
+           align_up(pl.ptask_words * 8) + align_up(pl.meta_bytes);
 }
 
 // Admission (DeviceConfig::is_out_of_memory, join_provider.cuh:148-150): the
@@ -315,29 +303,20 @@ Err admit(vj_ctx *c, const Plan &pl, uint64_t extra) {
         return Err::make(VJ_ERR_BUDGET, "Not enough memory for swap tables and hashes: need " + std::to_string(need) +
                                             " bytes, budget " + std::to_string(c->budget));
     // drop larger-than-needed arenas when keeping them would break the budget
-    uint64_t keep = 0;
-    keep += std::max<uint64_t>(c->tuples.bytes, pl.tuples_bytes) + std::max<uint64_t>(c->hist.bytes, pl.hist_words * 4) +
-            std::max<uint64_t>(c->base.bytes, pl.base_words * 4) + std::max<uint64_t>(c->p2pre.bytes, pl.p2pre_words * 4) +
-            std::max<uint64_t>(c->ptask.bytes, pl.ptask_words * 8) +
-            std::max<uint64_t>(c->status.bytes, pl.status_words * 8) + std::max<uint64_t>(c->segs.bytes, pl.segc_words * 8) +
-            std::max<uint64_t>(c->meta.bytes, pl.meta_bytes);
+    const uint64_t keep = std::max<uint64_t>(c->tuples.bytes, pl.tuples_bytes) +
+                          std::max<uint64_t>(c->counts.bytes, pl.counts_words * 4) +
+                          std::max<uint64_t>(c->base.bytes, pl.base_words * 4) +
+                          std::max<uint64_t>(c->ptask.bytes, pl.ptask_words * 8) +
+                          std::max<uint64_t>(c->meta.bytes, pl.meta_bytes);
     if (keep + extra > c->budget) free_all(c);
     return Err::ok();
 }
 
 Err allocate(vj_ctx *c, Plan &pl) {
     VJ_TRY(ensure(c->tuples, std::max<size_t>(pl.tuples_bytes, kAlign)));
-    const bool hist_grew = c->hist.bytes < pl.hist_words * 4;
-    VJ_TRY(ensure(c->hist, std::max<size_t>(pl.hist_words * 4, kAlign)));
-    if (hist_grew || c->hist_dirty) {
-        VJ_HIP_TRY(hipMemset(c->hist.p, 0, c->hist.bytes));
-        c->hist_dirty = false;
-    }
+    VJ_TRY(ensure(c->counts, std::max<size_t>(pl.counts_words * 4, kAlign)));
     VJ_TRY(ensure(c->base, std::max<size_t>(pl.base_words * 4, kAlign)));
-    VJ_TRY(ensure(c->p2pre, std::max<size_t>(pl.p2pre_words * 4, kAlign)));
     VJ_TRY(ensure(c->ptask, std::max<size_t>(pl.ptask_words * 8, kAlign)));
-    VJ_TRY(ensure(c->status, std::max<size_t>(pl.status_words * 8, kAlign)));
-    VJ_TRY(ensure(c->segs, std::max<size_t>(pl.segc_words * 8, kAlign)));
     VJ_TRY(ensure(c->meta, pl.meta_bytes));
     VJ_TRY(ensure_pinned(c, pl.meta_bytes));
     // tuple buffer pointers
@@ -365,31 +344,23 @@ PipeArgs make_args(vj_ctx *c, const Plan &pl) {
     memset(&a, 0, sizeof(a));
     a.jobs = reinterpret_cast<const JobDesc *>(m + pl.off_jobs);
     a.joins = reinterpret_cast<const JoinDesc *>(m + pl.off_joins);
+    a.colscan = reinterpret_cast<const ColScanDesc *>(m + pl.off_cs);
     a.hist_block0 = reinterpret_cast<const uint32_t *>(m + pl.off_hb0);
     a.p1_tile0 = reinterpret_cast<const uint32_t *>(m + pl.off_pt0);
-    a.job_p2_tiles = reinterpret_cast<uint32_t *>(m + pl.off_jp2);
-    a.p2_tile0 = reinterpret_cast<uint32_t *>(m + pl.off_p2t0);
+    a.p2_chunk0 = reinterpret_cast<const uint32_t *>(m + pl.off_pc0);
     a.join_tasks = reinterpret_cast<unsigned long long *>(m + pl.off_jtasks);
     a.task0 = reinterpret_cast<unsigned long long *>(m + pl.off_task0);
     a.out_count = reinterpret_cast<unsigned long long *>(m + pl.off_count);
     a.ctl = reinterpret_cast<Ctl *>(m + pl.off_ctl);
-    a.seg_ticket = reinterpret_cast<uint32_t *>(m + pl.off_segt);
-    a.p2_ticket = reinterpret_cast<uint32_t *>(m + pl.off_p2t);
-    a.segcnt = static_cast<uint32_t *>(c->segs.p);
-    a.segbase = static_cast<uint32_t *>(c->segs.p) + pl.segc_words;
-    a.zero_begin = m + pl.zero_begin;
-    a.zero_words = (pl.zero_end - pl.zero_begin) / 4;
-    a.hist = static_cast<uint32_t *>(c->hist.p);
+    a.counts = static_cast<uint32_t *>(c->counts.p);
     a.base = static_cast<uint32_t *>(c->base.p);
-    a.p2pre = static_cast<uint32_t *>(c->p2pre.p);
     a.ptask = static_cast<unsigned long long *>(c->ptask.p);
-    a.status = static_cast<unsigned long long *>(c->status.p);
-    a.status_words = pl.status_words;
     a.n_jobs = (uint32_t)pl.jobs.size();
     a.n_joins = (uint32_t)pl.joins.size();
+    a.n_colscan = (uint32_t)pl.colscan.size();
     a.total_hist_blocks = pl.total_hist_blocks;
     a.total_p1_tiles = pl.total_p1_tiles;
-    a.total_p2_tiles_max = pl.total_p2_tiles_max;
+    a.total_p2_chunks = pl.total_p2_chunks;
     a.max_hist_bins = pl.max_hist_bins;
     a.pass_tile = pass_tile(pl.kb);
     a.hash = pl.hash;
@@ -399,20 +370,20 @@ PipeArgs make_args(vj_ctx *c, const Plan &pl) {
 Err upload(vj_ctx *c, const Plan &pl, hipStream_t s) {
     char *h = static_cast<char *>(c->pinned);
     memcpy(h + pl.off_jobs, pl.jobs.data(), pl.jobs.size() * sizeof(JobDesc));
-    memcpy(h + pl.off_joins, pl.joins.data(), pl.joins.size() * sizeof(JoinDesc));
+    if (!pl.joins.empty()) memcpy(h + pl.off_joins, pl.joins.data(), pl.joins.size() * sizeof(JoinDesc));
+    if (!pl.colscan.empty()) memcpy(h + pl.off_cs, pl.colscan.data(), pl.colscan.size() * sizeof(ColScanDesc));
     memcpy(h + pl.off_hb0, pl.hist_block0.data(), pl.hist_block0.size() * 4);
     memcpy(h + pl.off_pt0, pl.p1_tile0.data(), pl.p1_tile0.size() * 4);
+    memcpy(h + pl.off_pc0, pl.p2_chunk0.data(), pl.p2_chunk0.size() * 4);
     VJ_HIP_TRY(hipMemcpyAsync(c->meta.p, h, pl.upload_bytes, hipMemcpyHostToDevice, s));
     return Err::ok();
 }
 
-#define VJ_LAUNCH(expr)                                                                         \
-    do {                                                                                        \
-        hipError_t e_ = (expr);                                                                 \
-        if (e_ != hipSuccess) {                                                                 \
-            c->hist_dirty = true;                                                               \
-            return Err::make(VJ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
-        }                                                                                       \
+#define VJ_LAUNCH(expr)                                                                       \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return Err::make(VJ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
     } while (0)
 
 // Launches every stage on `s` (no host synchronisation).

@@ -1,17 +1,17 @@
 // The join pipeline's kernels (gfx950). One pipeline runs any number of joins;
 // every kernel draws its tiles from all of them.
 //
-//   k_prepare   reset control words and per-join counters               (1 block)
-//   k_hist      2D radix histogram (pass-1 x pass-2 digit) per job: LDS
-//               counters, one global atomic per non-empty bin per block;
-//               also clears this pipeline's look-back status words
-//   k_scan      per join: exclusive scans of both histograms -> partition
-//               bases; pass-2 tile counts; build/probe task prefix
-//   k_scan2     prefixes across jobs / joins (pass-2 tiles, tasks)      (1 block)
-//   k_pass      one radix pass: tile -> per-wave LDS digit ranks -> decoupled
-//               look-back per digit over the earlier tiles of the same
-//               sequence -> LDS staging ordered by digit -> contiguous runs
-//               per digit written to HBM
+//   k_prepare   reset control words and per-join counters
+//   k_hist      per segment of a job: per-tile pass-1 digit prefixes, the
+//               segment's pass-1 and (pass-1, pass-2) digit counts (LDS)
+//   k_colscan   exclusive scans down the segment columns of those counts
+//   k_bases     per job: partition bases from the column totals
+//   k_tasks     per join: build/probe task prefix over partition pairs
+//   k_scan2     task prefix across joins                             (1 block)
+//   k_pass      one radix pass over a tile (pass 1) or a (digit, segment)
+//               chunk (pass 2): per-wave LDS digit ranks -> LDS staging
+//               ordered by digit -> contiguous runs per digit written at the
+//               precomputed destinations; no block waits on another
 //   k_probe     persistent; (join, partition, R-chunk, S-chunk) tasks from a
 //               device queue: open-addressing table of the R chunk in LDS,
 //               probe with the S chunk, 64-lane ballot compaction of matches
@@ -28,13 +28,6 @@
 namespace vj {
 
 namespace {
-
-__device__ __forceinline__ unsigned long long ld_status(const unsigned long long *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_status(unsigned long long *p, unsigned long long v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
@@ -124,80 +117,127 @@ template <typename T> __device__ void block_scan_array(const T *in, T *out, uint
 // ---------------------------------------------------------------------------
 
 __global__ void k_prepare(PipeArgs a) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.ctl->p2_total = 0;
+    if (threadIdx.x == 0) {
         a.ctl->task_ticket = 0;
         a.ctl->task_total = 0;
     }
-    uint32_t *z = static_cast<uint32_t *>(a.zero_begin);
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.zero_words;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        z[i] = 0;
+    for (uint32_t j = threadIdx.x; j < a.n_joins; j += blockDim.x) {
+        a.out_count[j] = 0;
+        a.join_tasks[j] = 0;
+    }
 }
 
-constexpr int kHistThreads = 1024;
+// One block per segment. The block sweeps the segment one pass-1 tile at a
+// time (all 1024 threads on one tile, coalesced), with one LDS atomic per row
+// into the tile's pass-1 digit counters and, for 2-pass jobs, one into the
+// segment's (pass-1, pass-2) histogram. Tile counters live in a ring of
+// kHistRing tiles; every kHistRing tiles the ring is folded into the running
+// in-segment prefix, which is stored per tile (tilepre).
+constexpr int kHistRing = kHistThreads / 64;
 
 template <int HP, int KW, int NC>
 __global__ __launch_bounds__(kHistThreads) void k_hist(PipeArgs a) {
-    extern __shared__ uint32_t lds_hist[];
-    const uint32_t b = blockIdx.x;
-    {
-        const uint64_t per = (a.status_words + gridDim.x - 1) / gridDim.x;
-        const uint64_t lo = (uint64_t)b * per;
-        const uint64_t hi = lo + per < a.status_words ? lo + per : a.status_words;
-        for (uint64_t i = lo + threadIdx.x; i < hi; i += kHistThreads) a.status[i] = 0;
-    }
+    constexpr int KB = KW * NC;
+    constexpr int T = pass_tile(KB);
+    constexpr int U = T / kHistThreads;
+    static_assert(U * kHistThreads == T, "a tile is a whole number of block sweeps");
+    extern __shared__ uint32_t lds[];
+    const uint32_t b = blockIdx.x, tid = threadIdx.x;
     const uint32_t j = upper_index(a.hist_block0, a.n_jobs, b);
     const JobDesc &job = a.jobs[j];
-    const uint32_t nb = 1u << (job.b1 + job.b2);
-    for (uint32_t i = threadIdx.x; i < nb; i += kHistThreads) lds_hist[i] = 0;
+    const uint32_t g = b - a.hist_block0[j];
+    const uint32_t n1 = 1u << job.b1, nb = 1u << (job.b1 + job.b2);
+    const bool two = job.b2 > 0;
+    uint32_t *h2 = lds;                    // [nb] (2-pass jobs)
+    uint32_t *run = lds + (two ? nb : 0);  // [n1] running in-segment prefix
+    uint32_t *th = run + n1;               // [kHistRing][n1] tile counts
+    for (uint32_t i = tid; i < (two ? nb : 0) + n1 * (1 + kHistRing); i += kHistThreads) lds[i] = 0;
     __syncthreads();
-    const uint32_t k = b - a.hist_block0[j];  // segment of this block
-    const uint64_t seg_rows = (uint64_t)job.seg_tiles * a.pass_tile;
-    const uint64_t lo = (uint64_t)k * seg_rows;
-    const uint64_t hi = lo + seg_rows < job.n ? lo + seg_rows : job.n;
-    const uint32_t m1 = (1u << job.b1) - 1, m2 = (1u << job.b2) - 1;
+    const uint32_t m1 = n1 - 1, m2 = (1u << job.b2) - 1, b2 = job.b2;
     const uint32_t s1 = job.shift0, s2 = job.shift0 + job.b1;
     const HashCfg hc = a.hash;
-    constexpr int U = 4;
-    uint64_t i = lo + threadIdx.x;
-    for (; i + (U - 1) * kHistThreads < hi; i += U * kHistThreads) {
-        Key<KW, NC> key[U];
+    const uint32_t t_begin = g * job.seg_tiles;
+    const uint32_t t_end = t_begin + job.seg_tiles < job.p1_tiles ? t_begin + job.seg_tiles : job.p1_tiles;
+    uint32_t *tilepre = a.counts + job.tilepre_off;
+    for (uint32_t t0 = t_begin; t0 < t_end; t0 += kHistRing) {
+        const uint32_t kt = t_end - t0 < (uint32_t)kHistRing ? t_end - t0 : (uint32_t)kHistRing;
+        for (uint32_t k = 0; k < kt; k++) {
+            uint32_t *thk = th + k * n1;
+            const uint32_t row_lo = (t0 + k) * T + tid;
+            Key<KW, NC> key[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) key[u] = load_key<KW, NC>(job.in, i + u * kHistThreads);
+            for (int u = 0; u < U; u++) {
+                const uint32_t i = row_lo + u * kHistThreads;
+                if (i < job.n) key[u] = load_key<KW, NC>(job.in, i);
+            }
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            uint64_t h = hash_key<HP, KW, NC>(key[u], hc);
-            uint32_t q = (((uint32_t)(h >> s1) & m1) << job.b2) | ((uint32_t)(h >> s2) & m2);
-            atomicAdd(&lds_hist[q], 1u);
+            for (int u = 0; u < U; u++) {
+                const uint32_t i = row_lo + u * kHistThreads;
+                if (i < job.n) {
+                    const uint64_t h = hash_key<HP, KW, NC>(key[u], hc);
+                    const uint32_t d1 = (uint32_t)(h >> s1) & m1;
+                    atomicAdd(&thk[d1], 1u);
+                    if (two) atomicAdd(&h2[(d1 << b2) | ((uint32_t)(h >> s2) & m2)], 1u);
+                }
+            }
         }
+        __syncthreads();
+        if (tid < n1) {
+            uint32_t acc = run[tid];
+            for (uint32_t k = 0; k < kt; k++) {
+                tilepre[(uint64_t)(t0 + k) * n1 + tid] = acc;
+                acc += th[k * n1 + tid];
+                th[k * n1 + tid] = 0;
+            }
+            run[tid] = acc;
+        }
+        __syncthreads();
     }
-    for (; i < hi; i += kHistThreads) {
-        Key<KW, NC> key = load_key<KW, NC>(job.in, i);
-        uint64_t h = hash_key<HP, KW, NC>(key, hc);
-        uint32_t q = (((uint32_t)(h >> s1) & m1) << job.b2) | ((uint32_t)(h >> s2) & m2);
-        atomicAdd(&lds_hist[q], 1u);
-    }
-    __syncthreads();
-    uint32_t *g = a.hist + job.hist_off;
-    for (uint32_t t = threadIdx.x; t < nb; t += kHistThreads) {
-        uint32_t c = lds_hist[t];
-        if (c) atomicAdd(&g[t], c);
-    }
-    // pass-1 digit counts of this segment (digit-major: [d1][segment])
-    const uint32_t n1 = 1u << job.b1, n2 = 1u << job.b2;
-    for (uint32_t d1 = threadIdx.x; d1 < n1; d1 += kHistThreads) {
-        uint32_t c = 0;
-        for (uint32_t d2 = 0; d2 < n2; d2++) c += lds_hist[(d1 << job.b2) | d2];
-        a.segcnt[job.segc_off + d1 * job.hist_blocks + k] = c;
+    uint32_t *m1row = a.counts + job.m1_off + (uint64_t)g * n1;
+    for (uint32_t d = tid; d < n1; d += kHistThreads) m1row[d] = run[d];
+    if (two) {
+        uint32_t *m2row = a.counts + job.m2_off + (uint64_t)g * nb;
+        for (uint32_t q = tid; q < nb; q += kHistThreads) m2row[q] = h2[q];
     }
 }
 
-// One block of 1024 threads per job: partition bases, pass-2 tile prefix.
-__global__ __launch_bounds__(1024) void k_scan(PipeArgs a) {
+// Exclusive scans down 256 columns of a [nseg + 1][ncols] count matrix (the
+// last row receives the column totals). 1024 threads = 256 columns x 4 row
+// groups; each thread sums its rows, the groups' partial sums are combined
+// in LDS, and the rows are rewritten with their prefixes.
+__global__ __launch_bounds__(1024) void k_colscan(PipeArgs a) {
+    __shared__ uint32_t part[4][256];
+    const ColScanDesc d = a.colscan[blockIdx.x];
+    const JobDesc &job = a.jobs[d.job];
+    const uint32_t ncols = d.which == 1 ? (1u << job.b1) : (1u << (job.b1 + job.b2));
+    uint32_t *m = a.counts + (d.which == 1 ? job.m1_off : job.m2_off);
+    const uint32_t c = d.col0 + (threadIdx.x & 255), rg = threadIdx.x >> 8;
+    const uint32_t rows = job.nseg, per = (rows + 3) / 4;
+    const uint32_t lo = rg * per < rows ? rg * per : rows, hi = lo + per < rows ? lo + per : rows;
+    const bool col_ok = c < ncols;
+    uint32_t sum = 0;
+    if (col_ok)
+        for (uint32_t r = lo; r < hi; r++) sum += m[(uint64_t)r * ncols + c];
+    part[rg][threadIdx.x & 255] = sum;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (uint32_t k = 0; k < rg; k++) pre += part[k][threadIdx.x & 255];
+    if (col_ok) {
+        for (uint32_t r = lo; r < hi; r++) {
+            const uint64_t at = (uint64_t)r * ncols + c;
+            const uint32_t v = m[at];
+            m[at] = pre;
+            pre += v;
+        }
+        if (rg == 3) m[(uint64_t)rows * ncols + c] = pre;
+    }
+}
+
+// One block of 1024 threads per job: partition bases = exclusive scan of the
+// column totals (the last row of m2, or of m1 for one pass).
+__global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
     __shared__ uint32_t sh32[20];
-    const uint32_t jj = blockIdx.x;
-    const JobDesc &job = a.jobs[jj];
+    const JobDesc &job = a.jobs[blockIdx.x];
     uint32_t *base = a.base + job.base_off;
     if (job.b1 == 0) {
         if (threadIdx.x == 0) {
@@ -207,37 +247,12 @@ __global__ __launch_bounds__(1024) void k_scan(PipeArgs a) {
         return;
     }
     const uint32_t nb = 1u << (job.b1 + job.b2);
-    uint32_t *hist = a.hist + job.hist_off;
-    block_scan_array<uint32_t>(hist, base, nb, sh32);
-    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) hist[i] = 0;  // ready for the next pipeline
-    if (job.b2 > 0) {
-        const uint32_t n1 = 1u << job.b1;
-        uint32_t *pre = a.p2pre + job.p2pre_off;
-        for (uint32_t d1 = threadIdx.x; d1 < n1; d1 += blockDim.x) {
-            const uint32_t cnt = base[(d1 + 1) << job.b2] - base[d1 << job.b2];
-            pre[d1] = (cnt + a.pass_tile - 1) / a.pass_tile;
-        }
-        __syncthreads();
-        block_scan_array<uint32_t>(pre, pre, n1, sh32);
-        if (threadIdx.x == 0) a.job_p2_tiles[jj] = pre[n1];
+    if (job.n == 0) {  // an empty job has no count rows
+        for (uint32_t q = threadIdx.x; q <= nb; q += blockDim.x) base[q] = 0;
+        return;
     }
-    // output base of every (pass-1 digit, segment): one wave scans one digit's segments
-    const uint32_t nseg = job.hist_blocks, n1 = 1u << job.b1;
-    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    const uint32_t per = (nseg + 63) / 64;
-    for (uint32_t d1 = wv; d1 < n1; d1 += nwv) {
-        const uint32_t *cnt = a.segcnt + job.segc_off + d1 * nseg;
-        uint32_t *sb = a.segbase + job.segc_off + d1 * nseg;
-        const uint32_t lo = lane * per < nseg ? lane * per : nseg, hi = lo + per < nseg ? lo + per : nseg;
-        uint32_t sum = 0;
-        for (uint32_t g = lo; g < hi; g++) sum += cnt[g];
-        uint32_t pre = wave_incl_scan(sum) - sum + base[d1 << job.b2];
-        for (uint32_t g = lo; g < hi; g++) {
-            const uint32_t c = cnt[g];
-            sb[g] = pre;
-            pre += c;
-        }
-    }
+    const uint32_t *tot = a.counts + (job.b2 ? job.m2_off : job.m1_off) + (uint64_t)job.nseg * nb;
+    block_scan_array<uint32_t>(tot, base, nb, sh32);
 }
 
 // One block of 1024 threads per join: build/probe tasks = R chunks x S chunks
@@ -268,14 +283,9 @@ __global__ void k_reset_probe(PipeArgs a) {
 }
 
 __global__ __launch_bounds__(1024) void k_scan2(PipeArgs a) {
-    __shared__ uint32_t sh32[20];
     __shared__ unsigned long long sh64[20];
-    block_scan_array<uint32_t>(a.job_p2_tiles, a.p2_tile0, a.n_jobs, sh32);
     block_scan_array<unsigned long long>(a.join_tasks, a.task0, a.n_joins, sh64);
-    if (threadIdx.x == 0) {
-        a.ctl->p2_total = a.p2_tile0[a.n_jobs];
-        a.ctl->task_total = a.task0[a.n_joins];
-    }
+    if (threadIdx.x == 0) a.ctl->task_total = a.task0[a.n_joins];
 }
 
 // ---------------------------------------------------------------------------
@@ -286,17 +296,16 @@ struct PassTile {
     TView src;
     void *dst_cols[4];
     uint32_t *dst_ids;
-    const uint32_t *dbase;  // digit d's output base is dbase[d * dstride]
-    unsigned long long *seq;  // status of tile 0 of this tile's sequence
-    uint32_t dstride;
-    uint32_t lo, hi;        // rows of src
+    uint32_t lo, hi;        // rows of src (pass 2: the whole chunk)
     uint32_t bins, shift;
-    uint32_t local;         // tile index inside its sequence
+    uint32_t job, local;    // job and tile (pass 1) / chunk (pass 2) index inside it
     uint32_t valid;
 };
 
+// Pass 1: one tile of a job. Pass 2: one (pass-1 digit, segment) chunk, in
+// tile-sized steps; a running count per digit carries the destination.
 template <int HP, int KW, int NC>
-__global__ __launch_bounds__(kPassThreads) void k_pass(PipeArgs a, int pass) {
+__global__ __launch_bounds__(kPassThreads, 6) void k_pass(PipeArgs a, int pass) {
     using KT = typename KeyT<KW>::T;
     constexpr int KB = KW * NC;
     constexpr int ITEMS = pass_items(KB);
@@ -307,171 +316,154 @@ __global__ __launch_bounds__(kPassThreads) void k_pass(PipeArgs a, int pass) {
     uint32_t *sids = reinterpret_cast<uint32_t *>(smem);                   // [T] (after the keys are out)
     uint32_t *wcnt = reinterpret_cast<uint32_t *>(smem + STAGE);           // [kPassWaves][256]
     uint32_t *loc = wcnt + kPassWaves * 256;                               // [256]
-    uint32_t *gdst = loc + 256;                                            // [256]
+    uint32_t *gdst = loc + 256;                                            // [256] next destination row
     uint8_t *sdig = reinterpret_cast<uint8_t *>(gdst + 256);               // [T]
     __shared__ PassTile ti;
     __shared__ uint32_t sh_scan[kPassWaves + 1];
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (uint32_t i = tid; i < kPassWaves * 256; i += kPassThreads) wcnt[i] = 0;
     if (w == 0) {
-        // Blocks map statically onto look-back sequences (pass 1: a segment of a
-        // job; pass 2: a pass-1 partition of a job), as many blocks as the
-        // sequence has tiles; inside a sequence the order comes from that
-        // sequence's own ticket, so a tile only waits on tiles whose blocks
-        // already run, and no counter is shared by more than one sequence.
-        const uint32_t b = blockIdx.x;
-        uint32_t valid, j, local;
-        if (pass == 1) {
-            valid = b < a.total_p1_tiles;
-            j = valid ? wave_upper_index(a.p1_tile0, a.n_jobs, b) : 0;
-            local = valid ? b - a.p1_tile0[j] : 0;
-        } else {
-            valid = b < a.ctl->p2_total;
-            j = valid ? wave_upper_index(a.p2_tile0, a.n_jobs, b) : 0;
-            local = valid ? b - a.p2_tile0[j] : 0;
-        }
-        const JobDesc &job = a.jobs[j];
-        const uint32_t *pre = a.p2pre + job.p2pre_off;
-        uint32_t seq = 0;  // segment (pass 1) or pass-1 partition (pass 2)
-        if (valid) seq = pass == 1 ? local / job.seg_tiles : wave_upper_index(pre, 1u << job.b1, local);
+        // Workgroups are dispatched round-robin over the XCDs; give each XCD a
+        // contiguous range of tiles so that the runs adjacent tiles write to
+        // one partition (which share cache lines) meet in the same L2.
+        const uint32_t *pfx = pass == 1 ? a.p1_tile0 : a.p2_chunk0;
+        const uint32_t total = pass == 1 ? a.total_p1_tiles : a.total_p2_chunks;
+        const uint32_t per_xcd = (total + kXcds - 1) / kXcds;
+        const uint32_t b = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+        const uint32_t valid = b < total;
+        const uint32_t j = valid ? wave_upper_index(pfx, a.n_jobs, b) : 0;
         if (lane == 0) {
             ti.valid = valid;
             if (valid) {
-                const uint32_t *base = a.base + job.base_off;
+                const JobDesc &job = a.jobs[j];
+                const uint32_t local = b - pfx[j];
+                ti.job = j;
+                ti.local = local;
                 if (pass == 1) {
-                    const uint32_t lt = atomicAdd(&a.seg_ticket[job.seg_off + seq], 1u);
-                    const uint32_t tile = seq * job.seg_tiles + lt;
                     ti.src = job.in;
                     for (int c = 0; c < 4; c++) ti.dst_cols[c] = job.b2 ? job.mid_cols[c] : job.fin_cols[c];
                     ti.dst_ids = job.b2 ? job.mid_ids : job.fin_ids;
-                    ti.dbase = a.segbase + job.segc_off + seq;
-                    ti.dstride = job.hist_blocks;
-                    ti.lo = tile * T;
+                    ti.lo = local * T;
                     ti.hi = ti.lo + T < job.n ? ti.lo + T : job.n;
                     ti.bins = 1u << job.b1;
                     ti.shift = job.shift0;
-                    ti.seq = a.status + job.p1_stat_off + (uint64_t)seq * job.seg_tiles * (1u << job.b1);
-                    ti.local = lt;
                 } else {
-                    const uint32_t d1 = seq;
-                    const uint32_t lt = atomicAdd(&a.p2_ticket[job.p2pre_off + d1], 1u);
+                    const uint32_t n1 = 1u << job.b1;
+                    const uint32_t d1 = local / job.nseg, g = local - d1 * job.nseg;
+                    const uint32_t p = a.base[job.base_off + (d1 << job.b2)];
+                    const uint32_t *m1 = a.counts + job.m1_off;
                     for (int c = 0; c < 4; c++) ti.src.cols[c] = job.mid_cols[c];
                     ti.src.rowids = job.mid_ids;
                     ti.src.n = job.n;
                     for (int c = 0; c < 4; c++) ti.dst_cols[c] = job.fin_cols[c];
                     ti.dst_ids = job.fin_ids;
-                    ti.dbase = base + (d1 << job.b2);
-                    ti.dstride = 1;
-                    const uint32_t p_lo = base[d1 << job.b2], p_hi = base[(d1 + 1) << job.b2];
-                    ti.lo = p_lo + lt * T;
-                    ti.hi = ti.lo + T < p_hi ? ti.lo + T : p_hi;
+                    ti.lo = p + m1[(uint64_t)g * n1 + d1];
+                    ti.hi = p + m1[(uint64_t)(g + 1) * n1 + d1];
                     ti.bins = 1u << job.b2;
                     ti.shift = job.shift0 + job.b1;
-                    ti.seq = a.status + job.p2_stat_off + (uint64_t)pre[d1] * (1u << job.b2);
-                    ti.local = lt;
                 }
             }
         }
     }
     __syncthreads();
     if (!ti.valid) return;
+    if (tid < ti.bins) {  // first destination row of every digit
+        const JobDesc &job = a.jobs[ti.job];
+        const uint32_t *base = a.base + job.base_off;
+        if (pass == 1) {
+            const uint32_t n1 = 1u << job.b1, t = ti.local, g = t / job.seg_tiles;
+            gdst[tid] = base[tid << job.b2] + a.counts[job.m1_off + (uint64_t)g * n1 + tid] +
+                        a.counts[job.tilepre_off + (uint64_t)t * n1 + tid];
+        } else {
+            const uint32_t nb = 1u << (job.b1 + job.b2);
+            const uint32_t d1 = ti.local / job.nseg, g = ti.local - d1 * job.nseg;
+            const uint32_t q = (d1 << job.b2) | tid;
+            gdst[tid] = base[q] + a.counts[job.m2_off + (uint64_t)g * nb + q];
+        }
+    }
 
-    const uint32_t lo = ti.lo, hi = ti.hi, bins = ti.bins, shift = ti.shift, mask = bins - 1;
+    const uint32_t bins = ti.bins, shift = ti.shift, mask = bins - 1;
     const TView src = ti.src;
     const HashCfg hc = a.hash;
+    for (uint32_t lo = ti.lo; lo < ti.hi; lo += T) {
+        const uint32_t hi = lo + T < ti.hi ? lo + T : ti.hi;
+        for (uint32_t i = tid; i < kPassWaves * 256; i += kPassThreads) wcnt[i] = 0;
+        __syncthreads();
 
-    // 1. load ITEMS rows per lane (wave w owns a contiguous 64*ITEMS segment), rank by digit
-    Key<KW, NC> keys[ITEMS];
-    uint32_t ids[ITEMS];
-    uint32_t pk[ITEMS];  // digit | rank << 8, or ~0 for no row
-    const uint32_t wbase = lo + w * 64 * ITEMS + lane;
+        // 1. load ITEMS rows per lane (wave w owns a contiguous 64*ITEMS segment), rank by digit
+        Key<KW, NC> keys[ITEMS];
+        uint32_t ids[ITEMS];
+        uint32_t pk[ITEMS];  // digit | rank << 8, or ~0 for no row
+        const uint32_t wbase = lo + w * 64 * ITEMS + lane;
 #pragma unroll
-    for (int it = 0; it < ITEMS; it++) {
-        const uint32_t i = wbase + it * 64;
-        if (i < hi) {
-            keys[it] = load_key<KW, NC>(src, i);
-            ids[it] = src.rowids ? src.rowids[i] : i;
-        }
-    }
-#pragma unroll
-    for (int it = 0; it < ITEMS; it++) {
-        const uint32_t i = wbase + it * 64;
-        pk[it] = ~0u;
-        if (i < hi) {
-            uint64_t h = hash_key<HP, KW, NC>(keys[it], hc);
-            uint32_t d = (uint32_t)(h >> shift) & mask;
-            uint32_t r = atomicAdd(&wcnt[w * 256 + d], 1u);
-            pk[it] = d | (r << 8);
-        }
-    }
-    __syncthreads();
-
-    // 2. per digit: wave prefixes, tile count, local offsets, look-back
-    uint32_t cnt = 0;
-    if (tid < bins) {
-#pragma unroll
-        for (int ww = 0; ww < kPassWaves; ww++) {
-            uint32_t c = wcnt[ww * 256 + tid];
-            wcnt[ww * 256 + tid] = cnt;
-            cnt += c;
-        }
-    }
-    uint32_t total;
-    const uint32_t my_loc = block_excl_scan<uint32_t>(cnt, sh_scan, &total);
-    if (tid < bins) {
-        loc[tid] = my_loc;
-        unsigned long long *my = ti.seq + (uint64_t)ti.local * bins + tid;
-        uint32_t excl = 0;
-        if (ti.local == 0) {
-            st_status(my, kFlagIncl | cnt);
-        } else {
-            st_status(my, kFlagAgg | cnt);
-            const unsigned long long *p = my - bins;
-            while (true) {
-                unsigned long long v = ld_status(p);
-                while ((v >> 62) == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    v = ld_status(p);
-                }
-                excl += (uint32_t)v;
-                if ((v >> 62) == 2) break;
-                p -= bins;
+        for (int it = 0; it < ITEMS; it++) {
+            const uint32_t i = wbase + it * 64;
+            if (i < hi) {
+                keys[it] = load_key<KW, NC>(src, i);
+                ids[it] = src.rowids ? src.rowids[i] : i;
             }
-            st_status(my, kFlagIncl | (unsigned long long)(excl + cnt));
         }
-        gdst[tid] = ti.dbase[tid * ti.dstride] + excl - my_loc;
-    }
-    __syncthreads();
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++) {
+            const uint32_t i = wbase + it * 64;
+            const bool valid = i < hi;
+            uint32_t d = 0;
+            if (valid) d = (uint32_t)(hash_key<HP, KW, NC>(keys[it], hc) >> shift) & mask;
+            const uint32_t r = valid ? atomicAdd(&wcnt[w * 256 + d], 1u) : 0u;
+            pk[it] = valid ? (d | (r << 8)) : ~0u;
+        }
+        __syncthreads();
 
-    // 3. stage the key columns in LDS ordered by digit; consecutive slots of one
-    //    digit go to consecutive output rows
-    uint32_t slot[ITEMS];
+        // 2. per digit: wave prefixes, tile count, local offsets
+        uint32_t cnt = 0;
+        if (tid < bins) {
 #pragma unroll
-    for (int it = 0; it < ITEMS; it++) {
-        slot[it] = ~0u;
-        if (pk[it] != ~0u) {
-            const uint32_t d = pk[it] & 255u;
-            slot[it] = loc[d] + wcnt[w * 256 + d] + (pk[it] >> 8);
-#pragma unroll
-            for (int c = 0; c < NC; c++) skeys[c * T + slot[it]] = keys[it].c[c];
-            sdig[slot[it]] = (uint8_t)d;
+            for (int ww = 0; ww < kPassWaves; ww++) {
+                uint32_t c = wcnt[ww * 256 + tid];
+                wcnt[ww * 256 + tid] = cnt;
+                cnt += c;
+            }
         }
-    }
-    __syncthreads();
-    const uint32_t n = hi - lo;
-    for (uint32_t k = tid; k < n; k += kPassThreads) {
-        const uint32_t dst = gdst[sdig[k]] + k;
+        uint32_t total;
+        const uint32_t my_loc = block_excl_scan<uint32_t>(cnt, sh_scan, &total);
+        if (tid < bins) loc[tid] = my_loc;
+        __syncthreads();
+
+        // 3. stage the key columns in LDS ordered by digit; consecutive slots of
+        //    one digit go to consecutive output rows
+        uint32_t slot[ITEMS];
 #pragma unroll
-        for (int c = 0; c < NC; c++) static_cast<KT *>(ti.dst_cols[c])[dst] = skeys[c * T + k];
-    }
-    __syncthreads();
-    // 4. the row ids, through the same staging buffer
+        for (int it = 0; it < ITEMS; it++) {
+            slot[it] = ~0u;
+            if (pk[it] != ~0u) {
+                const uint32_t d = pk[it] & 255u;
+                slot[it] = loc[d] + wcnt[w * 256 + d] + (pk[it] >> 8);
 #pragma unroll
-    for (int it = 0; it < ITEMS; it++)
-        if (slot[it] != ~0u) sids[slot[it]] = ids[it];
-    __syncthreads();
-    for (uint32_t k = tid; k < n; k += kPassThreads) ti.dst_ids[gdst[sdig[k]] + k] = sids[k];
+                for (int c = 0; c < NC; c++) skeys[c * T + slot[it]] = keys[it].c[c];
+                sdig[slot[it]] = (uint8_t)d;
+            }
+        }
+        __syncthreads();
+        const uint32_t n = hi - lo;
+        for (uint32_t k = tid; k < n; k += kPassThreads) {
+            const uint32_t d = sdig[k];
+            const uint32_t dst = gdst[d] + (k - loc[d]);
+#pragma unroll
+            for (int c = 0; c < NC; c++) static_cast<KT *>(ti.dst_cols[c])[dst] = skeys[c * T + k];
+        }
+        __syncthreads();
+        // 4. the row ids, through the same staging buffer
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++)
+            if (slot[it] != ~0u) sids[slot[it]] = ids[it];
+        __syncthreads();
+        for (uint32_t k = tid; k < n; k += kPassThreads) {
+            const uint32_t d = sdig[k];
+            ti.dst_ids[gdst[d] + (k - loc[d])] = sids[k];
+        }
+        __syncthreads();
+        if (tid < bins) gdst[tid] += cnt;  // next step of a pass-2 chunk
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -544,12 +536,14 @@ struct Emitter {
 //   WALK_COUNT   probe, count matches                          (build chunk has duplicate keys)
 //   WALK_EMIT    probe again, write the matches                (duplicates)
 //   WALK_RECORD  probe, count, keep each row's match in registers (no duplicates: <= 1 match)
-//   WALK_REPLAY  write the kept matches, no probing
-// All walks visit rows in the same order, so the counts of the first walk
-// are the row counts of the second. Probing runs in rounds: each round reads
-// one table slot for every unfinished row of the lane, so kProbeUnroll
-// independent LDS chains are in flight per lane.
-enum { WALK_COUNT = 0, WALK_EMIT = 1, WALK_RECORD = 2, WALK_REPLAY = 3 };
+// replay_walk then writes the kept matches without probing. All walks visit
+// rows in the same order, so the counts of the first walk are the row counts
+// of the second. Probing runs in rounds: each round reads one table slot for
+// every unfinished row of the lane, so kProbeUnroll independent LDS chains
+// are in flight per lane. The S keys of the next iteration are loaded before
+// the current one is probed (the caller loads iteration 0), so one HBM round
+// trip overlaps each iteration's probing (where registers allow, see PF).
+enum { WALK_COUNT = 0, WALK_EMIT = 1, WALK_RECORD = 2 };
 
 template <int N> __device__ __forceinline__ uint32_t res_get(const uint32_t (&r)[N], int k) {
     return (r[k >> 1] >> ((k & 1) * 16)) & 0xFFFFu;
@@ -561,51 +555,54 @@ template <int N> __device__ __forceinline__ void res_set(uint32_t (&r)[N], int k
 constexpr int kProbeIters = kProbeChunk / (kProbeThreads * kProbeUnroll);
 static_assert(kProbeIters * kProbeThreads * kProbeUnroll == kProbeChunk, "probe chunk shape");
 
+// First S row of this lane in iteration `it` of a walk.
+__device__ __forceinline__ uint32_t walk_row(const ProbeTask &t, int it) {
+    return t.s_lo + it * kProbeThreads * kProbeUnroll + (threadIdx.x >> 6) * 64 * kProbeUnroll + lane_id();
+}
+
+// Loads this lane's S keys of iteration `it` (rows past the chunk are left alone).
+template <int KW, int NC>
+__device__ __forceinline__ void walk_load(const ProbeTask &t, int it, Key<KW, NC> (&k)[kProbeUnroll]) {
+    const uint32_t row = walk_row(t, it), s_end = t.s_lo + t.s_n;
+#pragma unroll
+    for (int u = 0; u < kProbeUnroll; u++)
+        if (row + u * 64 < s_end) k[u] = load_key<KW, NC>(t.s, row + u * 64);
+}
+
 template <int MODE, int HP, int KW, int NC>
 __device__ __forceinline__ uint32_t probe_walk(const ProbeTask &t, const HashCfg &hc, uint32_t tag_shift,
                                                const typename KeyT<KW>::T *rk, const uint32_t *rid,
                                                const uint32_t *slots, uint32_t (&res)[kProbeIters * kProbeUnroll / 2],
-                                               Emitter &em) {
+                                               Key<KW, NC> (&nk)[kProbeUnroll], Emitter &em) {
     constexpr int CAP = build_cap(KW * NC);
     constexpr int U = kProbeUnroll;
-    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    // Next-iteration loads double the key registers; they are used where the
+    // kernel stays within its register budget (single 8-byte key, FNV / XOR).
+    constexpr bool PF = MODE != WALK_EMIT && KW * NC <= 8 && HP != HP_GENERIC;
     const uint32_t smask = t.nslots - 1, shift = t.shift, ob1 = t.one_based;
     const bool dup = t.dup != 0;
-    const uint32_t s_lo = t.s_lo, s_end = t.s_lo + t.s_n;
+    const uint32_t s_end = t.s_lo + t.s_n;
     uint32_t total = 0;
 #pragma unroll
     for (int it = 0; it < kProbeIters; it++) {
-        const uint32_t base = s_lo + it * kProbeThreads * U + w * 64 * U;
-        if (base >= s_end) break;  // wave-uniform
-        if (MODE == WALK_REPLAY) {
-            uint32_t sid[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const uint32_t i = base + u * 64 + lane;
-                sid[u] = 0;
-                if (i < s_end && res_get(res, it * U + u) != kNoMatch) sid[u] = t.s.rowids ? t.s.rowids[i] : i;
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const uint32_t o = res_get(res, it * U + u);
-                const bool m = o != kNoMatch;
-                em.put(m, (m ? rid[o] : 0u) + ob1, sid[u] + ob1, t);
-            }
-            continue;
-        }
+        const uint32_t row = walk_row(t, it);
+        if (row - lane_id() >= s_end) break;  // wave-uniform
         Key<KW, NC> sk[U];
         uint32_t sid[U], sl[U], tag[U];
         uint32_t act = 0;  // bit u: row u of this lane still probing
+        if (!PF && it > 0) walk_load<KW, NC>(t, it, nk);
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const uint32_t i = base + u * 64 + lane;
+            const uint32_t i = row + u * 64;
+            sk[u] = nk[u];
+            sid[u] = 0;
             if (MODE == WALK_RECORD) res_set(res, it * U + u, kNoMatch);
             if (i < s_end) {
-                sk[u] = load_key<KW, NC>(t.s, i);
                 if (MODE == WALK_EMIT) sid[u] = t.s.rowids ? t.s.rowids[i] : i;
                 act |= 1u << u;
             }
         }
+        if (PF && it + 1 < kProbeIters) walk_load<KW, NC>(t, it + 1, nk);
 #pragma unroll
         for (int u = 0; u < U; u++) {
             sl[u] = 0;
@@ -653,8 +650,49 @@ __device__ __forceinline__ uint32_t probe_walk(const ProbeTask &t, const HashCfg
     return total;
 }
 
+// Writes the matches a WALK_RECORD walk kept. The S row ids of a group of G
+// iterations are loaded before any is written, so their round trips overlap.
+template <int G>
+__device__ __forceinline__ void replay_walk(const ProbeTask &t, const uint32_t *rid,
+                                            const uint32_t (&res)[kProbeIters * kProbeUnroll / 2], Emitter &em) {
+    constexpr int U = kProbeUnroll;
+    static_assert(kProbeIters % G == 0, "replay groups");
+    const uint32_t ob1 = t.one_based, s_end = t.s_lo + t.s_n;
+#pragma unroll
+    for (int g = 0; g < kProbeIters; g += G) {
+        if (walk_row(t, g) - lane_id() >= s_end) break;  // wave-uniform
+        uint32_t sid[G * U];
+#pragma unroll
+        for (int it = 0; it < G; it++) {
+            const uint32_t row = walk_row(t, g + it);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint32_t i = row + u * 64;
+                sid[it * U + u] = i;
+                if (t.s.rowids && i < s_end && res_get(res, (g + it) * U + u) != kNoMatch)
+                    sid[it * U + u] = t.s.rowids[i];
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < G; it++) {
+            if (walk_row(t, g + it) - lane_id() >= s_end) break;  // wave-uniform
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint32_t o = res_get(res, (g + it) * U + u);
+                const bool m = o != kNoMatch;
+                em.put(m, (m ? rid[o] : 0u) + ob1, sid[it * U + u] + ob1, t);
+            }
+        }
+    }
+}
+
+// Waves per SIMD the probe kernel is compiled for: 6 (80 VGPRs, three blocks
+// per CU) for single 8-byte keys with FNV / XOR hashing; wider keys and the
+// generic hash path get 4 (128 VGPRs) so they do not spill.
+template <int HP, int KB> constexpr int probe_waves_per_simd() { return KB <= 8 && HP != HP_GENERIC ? 6 : 4; }
+
 template <int HP, int KW, int NC>
-__global__ __launch_bounds__(kProbeThreads, 6) void k_probe(PipeArgs a) {
+__global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>())) void k_probe(PipeArgs a) {
     using KT = typename KeyT<KW>::T;
     constexpr int KB = KW * NC;
     constexpr int CAP = build_cap(KB);
@@ -764,8 +802,10 @@ __global__ __launch_bounds__(kProbeThreads, 6) void k_probe(PipeArgs a) {
         // count (recording matches when the chunk has no duplicate keys), reserve
         // the task's output rows with one atomic, then write them
         const bool dup = t.dup != 0;
-        const uint32_t wc = dup ? probe_walk<WALK_COUNT, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, res, em)
-                                : probe_walk<WALK_RECORD, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, res, em);
+        Key<KW, NC> nk[kProbeUnroll];
+        walk_load<KW, NC>(t, 0, nk);
+        const uint32_t wc = dup ? probe_walk<WALK_COUNT, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, res, nk, em)
+                                : probe_walk<WALK_RECORD, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, res, nk, em);
         if (lane == 0) t.wave_count[w] = wc;
         __syncthreads();
         if (tid == 0) {
@@ -778,8 +818,13 @@ __global__ __launch_bounds__(kProbeThreads, 6) void k_probe(PipeArgs a) {
             em.pos = t.out_base;
             em.wn = 0;
             for (uint32_t k = 0; k < w; k++) em.pos += t.wave_count[k];
-            if (dup) probe_walk<WALK_EMIT, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, res, em);
-            else probe_walk<WALK_REPLAY, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, res, em);
+            if (dup) {
+                walk_load<KW, NC>(t, 0, nk);
+                probe_walk<WALK_EMIT, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, res, nk, em);
+            } else {
+                // row-id groups of four iterations where registers allow (single 8-byte key, FNV / XOR)
+                replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 4 : 1>(t, rid, res, em);
+            }
             em.finish(t);
         }
         __syncthreads();
@@ -824,9 +869,7 @@ size_t probe_lds_bytes(int kw, int nc) {
 }  // namespace
 
 hipError_t launch_prepare(const PipeArgs &a, hipStream_t s) {
-    uint64_t blocks = (a.zero_words + 1023) / 1024;
-    blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);
-    hipLaunchKernelGGL(k_prepare, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -840,13 +883,16 @@ hipError_t launch_hist(const PipeArgs &a, const LaunchShape &sh, hipStream_t s) 
 }
 
 hipError_t launch_scan(const PipeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_scan, dim3(a.n_jobs), dim3(1024), 0, s, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    hipError_t e;
+    if (a.n_colscan) {
+        hipLaunchKernelGGL(k_colscan, dim3(a.n_colscan), dim3(1024), 0, s, a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_bases, dim3(a.n_jobs), dim3(1024), 0, s, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.n_joins) {
         hipLaunchKernelGGL(k_tasks, dim3(a.n_joins), dim3(1024), 0, s, a);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
+        if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, s, a);
     return hipGetLastError();
@@ -858,8 +904,9 @@ hipError_t launch_reset_probe(const PipeArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_pass(const PipeArgs &a, const LaunchShape &sh, int pass, hipStream_t s) {
-    const uint32_t grid = pass == 1 ? a.total_p1_tiles : a.total_p2_tiles_max;
-    if (grid == 0) return hipSuccess;
+    const uint32_t total = pass == 1 ? a.total_p1_tiles : a.total_p2_chunks;
+    if (total == 0) return hipSuccess;
+    const uint32_t grid = (total + kXcds - 1) / kXcds * kXcds;
     const size_t lds = pass_lds_bytes(sh.kw, sh.nc);
     return dispatch(sh, [&]<int HP, int KW, int NC>() {
         hipLaunchKernelGGL((k_pass<HP, KW, NC>), dim3(grid), dim3(kPassThreads), lds, s, a, pass);
@@ -868,9 +915,15 @@ hipError_t launch_pass(const PipeArgs &a, const LaunchShape &sh, int pass, hipSt
 }
 
 int probe_blocks_per_cu(const LaunchShape &sh) {
-    const size_t lds = probe_lds_bytes(sh.kw, sh.nc) + sizeof(ProbeTask);
-    int by_lds = (int)((160 * 1024) / lds);
-    return by_lds < 1 ? 1 : (by_lds > 8 ? 8 : by_lds);
+    const size_t lds = probe_lds_bytes(sh.kw, sh.nc);
+    int blocks = 1;
+    (void)dispatch(sh, [&]<int HP, int KW, int NC>() {
+        int n = 0;
+        const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_probe<HP, KW, NC>, kProbeThreads, lds);
+        if (e == hipSuccess && n > 0) blocks = n;
+        return e;
+    });
+    return blocks > 8 ? 8 : blocks;
 }
 
 hipError_t launch_probe(const PipeArgs &a, const LaunchShape &sh, int blocks, hipStream_t s) {

@@ -5,6 +5,21 @@
 // hashed and radix-partitioned on the same digits; the build+probe stage then
 // draws (join, partition, R-chunk, S-chunk) tasks from a device work queue.
 //
+// Partitioning is count-then-scatter with exact destinations, so scatter
+// blocks never wait on each other:
+//  * A job's rows are cut into pass-1 tiles (T rows) grouped into segments.
+//    The histogram kernel walks each segment's tiles in order and records, per
+//    tile, the exclusive prefix of its pass-1 digit counts inside the segment
+//    (tilepre), and per segment the pass-1 digit counts (m1 row) and the
+//    (pass-1, pass-2) digit counts (m2 row).
+//  * Column scans over the segments turn m1 / m2 into per-segment offsets,
+//    and a scan over the column totals gives the partition bases.
+//  * Pass 1 writes tile t's rows of digit d1 at
+//      base[d1 << b2] + m1[seg(t)][d1] + tilepre[t][d1] + rank.
+//  * Pass 2 reads chunk (d1, g) = segment g's rows of pass-1 partition d1 (a
+//    contiguous range of the pass-1 output) and writes its rows of digit d2 at
+//      base[(d1 << b2) | d2] + m2[g][(d1 << b2) | d2] + rank.
+//
 // HBM layout per partitioned job: SoA copies of the key columns plus a uint32
 // row-id column, once after pass 1 ("mid", only for 2-pass jobs) and once
 // after the last pass ("fin"). Partition q of a job occupies rows
@@ -16,23 +31,21 @@
 
 namespace vj {
 
-constexpr int kPassThreads = 512;   // radix pass kernel
+constexpr int kPassThreads = 512;   // radix pass kernels
 constexpr int kPassWaves = kPassThreads / 64;
+constexpr int kHistThreads = 1024;  // histogram kernel
 constexpr int kProbeThreads = 512;  // build+probe kernel
 constexpr int kProbeWaves = kProbeThreads / 64;
-constexpr int kMaxPassBits = 8;   // digits per pass <= 256
-constexpr int kMaxHistBits = 14;  // 2D histogram in LDS: 2^14 x 4 B = 64 KiB
-constexpr int kOutBuf = 128;      // per-wave LDS output staging (pairs)
+constexpr int kMaxPassBits = 8;     // digits per pass <= 256
+constexpr int kMaxHistBits = 14;    // 2D histogram in LDS: 2^14 x 4 B = 64 KiB
+constexpr int kOutBuf = 128;        // per-wave LDS output staging (pairs)
+constexpr uint32_t kXcds = 8;       // accelerator dies (XCDs) of an MI355X
 
 // Tuples per pass tile, per build chunk and per probe chunk, by key bytes per row.
 __host__ __device__ constexpr int pass_items(int key_bytes) { return key_bytes <= 8 ? 8 : (key_bytes <= 16 ? 4 : 2); }
 __host__ __device__ constexpr int pass_tile(int key_bytes) { return kPassThreads * pass_items(key_bytes); }
 __host__ __device__ constexpr int build_cap(int key_bytes) { return key_bytes <= 8 ? 2048 : 1024; }
 constexpr int kProbeChunk = 16384;  // probe rows per build+probe task
-
-// Status word of the decoupled look-back: flag in bits 62..63, count in 0..31.
-constexpr unsigned long long kFlagAgg = 1ull << 62;
-constexpr unsigned long long kFlagIncl = 2ull << 62;
 
 struct JobDesc {
     TView in;                 // the caller's table
@@ -43,17 +56,13 @@ struct JobDesc {
     uint32_t n;
     uint32_t b1, b2;          // radix bits of pass 1 / pass 2 (b1 == 0: not partitioned)
     uint32_t shift0;          // digit of pass 1 = (h >> shift0) & (2^b1 - 1)
-    uint32_t hist_off;        // u32 offset of the 2^(b1+b2) histogram in the hist arena
     uint32_t base_off;        // u32 offset of the 2^(b1+b2)+1 partition bases in the base arena
-    uint32_t p2pre_off;       // u32 offset of the 2^b1+1 pass-2 tile prefix (2-pass jobs)
-    uint32_t hist_blocks;     // histogram blocks = pass-1 segments of this job
-    uint32_t seg_tiles;       // pass-1 tiles per segment (segment k = tiles [k*seg_tiles, (k+1)*seg_tiles))
-    uint32_t seg_off;         // offset of this job's segments in the per-segment ticket array
-    uint32_t segc_off;        // offset of this job's (digit, segment) counts / bases, digit-major
-    uint32_t p1_tiles;        // pass-1 tiles of this job
-    uint32_t p2_tiles_max;    // upper bound of pass-2 tiles
-    uint64_t p1_stat_off;     // u64 offset of pass-1 tile status (p1_tiles x 2^b1)
-    uint64_t p2_stat_off;     // u64 offset of pass-2 tile status (p2_tiles_max x 2^b2)
+    uint32_t p1_tiles;        // pass-1 tiles
+    uint32_t seg_tiles;       // pass-1 tiles per segment
+    uint32_t nseg;            // segments (= histogram blocks)
+    uint64_t tilepre_off;     // u32 offset of tilepre [p1_tiles][2^b1] in the count arena
+    uint64_t m1_off;          // u32 offset of m1 [nseg + 1][2^b1]
+    uint64_t m2_off;          // u32 offset of m2 [nseg + 1][2^(b1+b2)] (2-pass jobs)
 };
 
 struct JoinDesc {
@@ -67,10 +76,16 @@ struct JoinDesc {
     uint32_t one_based;
 };
 
+// One block of the column-scan kernel: 256 columns of one job's m1 or m2.
+struct ColScanDesc {
+    uint32_t job;
+    uint32_t which;  // 1: m1, 2: m2
+    uint32_t col0;
+    uint32_t pad;
+};
+
 // Control words, reset by k_prepare at the start of every pipeline.
 struct Ctl {
-    unsigned int p2_total;
-    unsigned int pad[3];
     unsigned long long task_ticket;
     unsigned long long task_total;
 };
@@ -78,30 +93,21 @@ struct Ctl {
 struct PipeArgs {
     const JobDesc *jobs;
     const JoinDesc *joins;
-    const uint32_t *hist_block0;   // [n_jobs + 1] prefix of hist_blocks (host)
-    const uint32_t *p1_tile0;      // [n_jobs + 1] prefix of p1_tiles (host)
-    uint32_t *job_p2_tiles;        // [n_jobs] (device)
-    uint32_t *p2_tile0;            // [n_jobs + 1] (device)
-    unsigned long long *join_tasks;  // [n_joins] (device)
-    unsigned long long *task0;       // [n_joins + 1] (device)
-    unsigned long long *out_count;   // [n_joins] (device)
-    uint32_t *seg_ticket;          // [total segments] pass-1 ticket per segment (device)
-    uint32_t *p2_ticket;           // [p2pre words] pass-2 ticket per pass-1 partition (device)
-    uint32_t *segcnt;              // [total segments x 2^b1] digit counts per segment, digit-major
-    uint32_t *segbase;             // same shape: output base of each (digit, segment)
-    void *zero_begin;              // per-pipeline counters cleared by k_prepare
-    uint64_t zero_words;
+    const ColScanDesc *colscan;     // [n_colscan]
+    const uint32_t *hist_block0;    // [n_jobs + 1] prefix of nseg (host)
+    const uint32_t *p1_tile0;       // [n_jobs + 1] prefix of p1_tiles (host)
+    const uint32_t *p2_chunk0;      // [n_jobs + 1] prefix of pass-2 chunks (host)
+    unsigned long long *join_tasks;   // [n_joins] (device)
+    unsigned long long *task0;        // [n_joins + 1] (device)
+    unsigned long long *out_count;    // [n_joins] (device)
     Ctl *ctl;
-    uint32_t *hist;
+    uint32_t *counts;               // count arena: tilepre / m1 / m2 of every job
     uint32_t *base;
-    uint32_t *p2pre;
     unsigned long long *ptask;
-    unsigned long long *status;
-    uint64_t status_words;         // zeroed by the histogram kernel
-    uint32_t n_jobs, n_joins;
-    uint32_t total_hist_blocks, total_p1_tiles, total_p2_tiles_max;
-    uint32_t max_hist_bins;        // LDS bins of the histogram kernel
-    uint32_t pass_tile;            // tuples per radix-pass tile (pass_tile(key bytes))
+    uint32_t n_jobs, n_joins, n_colscan;
+    uint32_t total_hist_blocks, total_p1_tiles, total_p2_chunks;
+    uint32_t max_hist_bins;         // LDS bins of the histogram kernel
+    uint32_t pass_tile;             // tuples per radix-pass tile (pass_tile(key bytes))
     HashCfg hash;
 };
 

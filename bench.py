@@ -69,6 +69,97 @@ def cpu_baseline(log_r, log_s, passes, bits1, bits2):
                       "oracle/vj_oracle.c, %.2f s)" % (log_r, log_s, passes, bits1, bits2, dt)}
 
 
+# Stage -> kernel (the HIP events around a stage bracket exactly this kernel, except
+# the histogram stage, which also holds the one-block k_prepare).
+STAGE_KERNEL = {"histogram": "k_hist", "pass1": "k_pass", "pass2": "k_pass", "build_probe": "k_probe"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "large_fk_traffic.txt")
+CALIB_BYTES = 1 << 30  # plain copy used to scale the FETCH_SIZE / WRITE_SIZE counters
+
+
+def _pmc_rows(path, counter):
+    """(dispatch id, kernel name, counter value in bytes) from a rocprofv3 --pmc
+    result: its sqlite database (counters_collection view) or counter_collection.csv."""
+    rows = []
+    if path.endswith(".db"):
+        import sqlite3
+        db = sqlite3.connect(path)
+        for d, name, c, v in db.execute("select dispatch_id, kernel_name, counter_name, value from counters_collection"):
+            if c == counter:
+                rows.append((int(d), name, float(v) * 1024.0))
+        db.close()
+    else:
+        import csv
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter:
+                    rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"]) * 1024.0))
+    rows.sort()
+    return rows
+
+
+def _stage_of_dispatches(rows):
+    """Maps each join-kernel dispatch to its stage; the two k_pass dispatches of a
+    join come in order (pass 1, then pass 2)."""
+    out, npass = [], 0
+    for _, name, val in rows:
+        if "k_hist" in name:
+            out.append(("histogram", val))
+        elif "k_pass" in name:
+            out.append(("pass1" if npass % 2 == 0 else "pass2", val))
+            npass += 1
+        elif "k_probe" in name and "reset" not in name:
+            out.append(("build_probe", val))
+    return out
+
+
+def traffic_from_pmc(fetch_csv, write_csv, workload):
+    """HBM bytes per launch of each stage's kernel from two rocprofv3 --pmc runs,
+    each counter scaled by what it reads for a plain 1 GiB device copy."""
+    import numpy as np
+    lines = ["# HBM traffic per launch, from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch),",
+             "# each scaled by counter(1 GiB device copy) / 1 GiB. Workload: " + workload,
+             "# made by: rocprofv3 --pmc FETCH_SIZE -d F -o run -- python3 bench.py --steps 1 --warmup 1 --no-cpu "
+             "--calib-copy",
+             "#          rocprofv3 --pmc WRITE_SIZE -d W -o run -- python3 bench.py --steps 1 --warmup 1 --no-cpu "
+             "--calib-copy",
+             "#          python3 bench.py --traffic-from F/run_results.db W/run_results.db"]
+    factors, per = {}, {}
+    for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        rows = _pmc_rows(path, counter)
+        copies = [v for _, n, v in rows if "copy" in n.lower()]
+        big = max(copies)
+        calib = float(np.median([v for v in copies if v > 0.5 * big]))
+        factors[counter] = CALIB_BYTES / calib
+        for st, v in _stage_of_dispatches(rows):
+            per.setdefault((st, counter), []).append(v)
+        lines.append("calib %s raw_bytes_for_1GiB_copy=%.0f scale=%.4f" % (counter, calib, factors[counter]))
+    for st in ("histogram", "pass1", "pass2", "build_probe"):
+        if (st, "FETCH_SIZE") not in per:
+            continue
+        fb = float(np.median(per[(st, "FETCH_SIZE")])) * factors["FETCH_SIZE"]
+        wb = float(np.median(per[(st, "WRITE_SIZE")])) * factors["WRITE_SIZE"]
+        lines.append("stage=%s kernel=%s fetch_bytes=%.0f write_bytes=%.0f traffic_bytes=%.0f dispatches=%d" %
+                     (st, STAGE_KERNEL[st], fb, wb, fb + wb, len(per[(st, "FETCH_SIZE")])))
+    return "\n".join(lines) + "\n"
+
+
+def read_traffic(stage, workload):
+    """Traffic per launch for `stage` from the committed profile summary, if it
+    was measured on this workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            text = f.read()
+    except OSError:
+        return None
+    if ("Workload: " + workload) not in text:
+        return None
+    for line in text.splitlines():
+        if line.startswith("stage=%s " % stage):
+            kv = dict(x.split("=", 1) for x in line.split())
+            return float(kv["traffic_bytes"])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,7 +171,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-log-r", type=int, default=23)
     ap.add_argument("--cpu-log-s", type=int, default=26)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the timed steps, copy 1 GiB device to device three times (counter calibration)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="no GPU: write the traffic summary from two rocprofv3 --pmc results (run_results.db or "
+                         "counter_collection.csv)")
     args = ap.parse_args()
+    workload_tag = "large_fk build 2^%d probe 2^%d passes %d" % (args.log_r, args.log_s, args.passes)
+    if args.traffic_from:
+        text = traffic_from_pmc(args.traffic_from[0], args.traffic_from[1], workload_tag)
+        with open(TRAFFIC_FILE, "w") as f:
+            f.write(text)
+        print(text, end="")
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the join shards by a top radix digit across GPUs "
               "(see DESIGN.md) but no multi-GPU run exists. Use --gpus 1.", file=sys.stderr)
@@ -139,6 +242,13 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     summ = ctx.summary()
+    if args.calib_copy:
+        x = torch.empty(CALIB_BYTES // 8, dtype=torch.int64, device="cuda").fill_(1)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
     ms_step = elapsed / args.steps * 1e3
     med = {k: float(np.median(v)) for k, v in stages.items()}
     sb = stage_bytes(nr, ns, m)
@@ -164,8 +274,9 @@ def main():
                                                                                    summ["passes"]),
                    "build_rows": nr, "probe_rows": ns, "matches": m, "passes": summ["passes"],
                    "bits": [summ["bits1"], summ["bits2"]], "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": "%s (%s)" % (STAGE_KERNEL[dominant], dominant),
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": read_traffic(dominant, workload_tag),
                      "algorithmic_bytes_per_launch": sb[dominant]},
         "join_roofline": {"algorithmic_bytes": B, "ms": med["total"], "achieved_GBps": join_gbs / 1e9,
                           "frac": join_gbs / HBM_PEAK},

@@ -160,6 +160,90 @@ def read_traffic(stage, workload):
     return None
 
 
+def _pair_checksums(outs_r, outs_s, counts):
+    """Order-independent checksum of each join's pairs: sum of r * 2^32 + s (mod 2^64)."""
+    import torch
+    sums = []
+    for r, s_, n in zip(outs_r, outs_s, counts):
+        rr = r[:n].to(torch.int64) & 0xFFFFFFFF
+        ss = s_[:n].to(torch.int64) & 0xFFFFFFFF
+        sums.append(int(((rr << 32) | ss).sum().item()) if n else 0)
+    return sums
+
+
+def batched_leg(ctx, n_joins, reps, seed=None):
+    """256 independent int32 joins of uneven sizes (build 2^U(10,20), probe 4-16x,
+    ~50 % match): one batched pipeline replayed from a hipGraph against the same
+    joins as sequential plain vj_join_into calls. Both produce every join's pairs;
+    counts and pair checksums must agree join by join."""
+    import numpy as np
+    import torch
+
+    import vjoin
+    import workloads
+    joins = workloads.batched(n_joins=n_joins, seed=seed)
+    builds, probes, outs_r, outs_s, caps, keep = [], [], [], [], [], []
+    rows = 0
+    for rcols, scols in joins:
+        dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+        keep += [dr, ds]
+        builds.append(vjoin.table(dr))
+        probes.append(vjoin.table(ds))
+        cap = len(scols[0])
+        rows += len(rcols[0]) + cap
+        outs_r.append(torch.empty(cap, dtype=torch.int32, device="cuda"))
+        outs_s.append(torch.empty(cap, dtype=torch.int32, device="cuda"))
+        caps.append(cap)
+    del joins
+    opts = vjoin.join_opts("fnv", 64, 64)
+    torch.cuda.synchronize()
+
+    def batch(graph):
+        rc, counts = ctx.join_batch(builds, probes, opts, outs_r, outs_s, caps, use_graph=graph)
+        if rc != 0:
+            raise RuntimeError("batch failed: %s %s" % (vjoin.STATUS.get(rc), ctx.last_error()))
+        return counts
+
+    def plain():
+        counts = []
+        for i in range(len(builds)):
+            rc, cnt = ctx.join_into(builds[i], probes[i], opts, outs_r[i], outs_s[i])
+            if rc != 0:
+                raise RuntimeError("join failed: %s %s" % (vjoin.STATUS.get(rc), ctx.last_error()))
+            counts.append(cnt)
+        return counts
+
+    # warm-up (allocates the arenas, captures the graph), then timed runs
+    c_plain = plain()
+    sum_plain = _pair_checksums(outs_r, outs_s, c_plain)
+    c_batch = batch(True)
+    sum_batch = _pair_checksums(outs_r, outs_s, c_batch)
+    t_plain, t_batch, t_eager = [], [], []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plain()
+        torch.cuda.synchronize()
+        t_plain.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        batch(True)
+        torch.cuda.synchronize()
+        t_batch.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        batch(False)
+        torch.cuda.synchronize()
+        t_eager.append(time.perf_counter() - t0)
+    c_last = batch(True)
+    same = c_plain == c_batch == c_last and sum_plain == sum_batch == _pair_checksums(outs_r, outs_s, c_last)
+    tp, tb, te = float(np.median(t_plain)), float(np.median(t_batch)), float(np.median(t_eager))
+    return {"workload": "%d int32 joins, build 2^U(10,20), probe U(4,16)x build, ~50%% match, FNV-1a-64" % n_joins,
+            "input_rows": rows, "matches": int(sum(c_plain)), "reps": reps,
+            "joins_per_s_batched_graph": n_joins / tb, "joins_per_s_batched_eager": n_joins / te,
+            "joins_per_s_plain": n_joins / tp, "speedup_vs_plain": tp / tb,
+            "ms_batched_graph": tb * 1e3, "ms_batched_eager": te * 1e3, "ms_plain": tp * 1e3,
+            "same_counts_and_checksums": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +255,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-log-r", type=int, default=23)
     ap.add_argument("--cpu-log-s", type=int, default=26)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--batched-joins", type=int, default=256)
+    ap.add_argument("--batched-reps", type=int, default=3)
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device three times (counter calibration)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -285,6 +372,11 @@ def main():
         "output_alloc_ms": t_alloc * 1e3,
         "parity": {"fk_every_probe_row_once_equal_keys": ok},
     }
+    if not args.no_batched:
+        del dr, ds, build, probe, out_r, out_s
+        torch.cuda.empty_cache()
+        print("bench: batched leg (%d joins)" % args.batched_joins, file=sys.stderr, flush=True)
+        out["batched"] = batched_leg(ctx, args.batched_joins, args.batched_reps)
     if not args.no_cpu:
         print("bench: cpu baseline (OpenMP oracle)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_log_r, args.cpu_log_s, summ["passes"], summ["bits1"],

@@ -220,6 +220,55 @@ def test_batch_of_uneven_joins(ctx):
         _assert_same(got, want, i)
 
 
+def _batch_inputs(joins):
+    import torch
+    builds, probes, outs_r, outs_s, caps, keep = [], [], [], [], [], []
+    for rcols, scols in joins:
+        dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+        keep += [dr, ds]
+        builds.append(vjoin.table(dr))
+        probes.append(vjoin.table(ds))
+        cap = len(scols[0])
+        outs_r.append(torch.empty(max(cap, 1), dtype=torch.int32, device="cuda"))
+        outs_s.append(torch.empty(max(cap, 1), dtype=torch.int32, device="cuda"))
+        caps.append(cap)
+    return builds, probes, outs_r, outs_s, caps, keep
+
+
+def test_batch_graph_replays_match_eager(ctx):
+    """The captured pipeline, replayed several times with other work and
+    synchronisations in between, gives the eager result every time; a batch of
+    another shape is captured anew, and the first shape still replays right."""
+    import torch
+    joins_a = workloads.batched(n_joins=16, log_min=8, log_max=14, seed=11)
+    joins_b = workloads.batched(n_joins=9, log_min=9, log_max=13, seed=12)
+    ia, ib = _batch_inputs(joins_a), _batch_inputs(joins_b)
+    opts = vjoin.join_opts()
+    want_a = [np_join(workloads.reference_rows(r), workloads.reference_rows(s)) for r, s in joins_a]
+    want_b = [np_join(workloads.reference_rows(r), workloads.reference_rows(s)) for r, s in joins_b]
+
+    def run(inp, want, use_graph, what):
+        builds, probes, outs_r, outs_s, caps, _ = inp
+        for o in outs_r + outs_s:
+            o.fill_(-1)
+        rc, counts = ctx.join_batch(builds, probes, opts, outs_r, outs_s, caps, use_graph=use_graph)
+        assert rc == 0, ctx.last_error()
+        for i in range(len(builds)):
+            _assert_same(_pairs(outs_r[i][:counts[i]], outs_s[i][:counts[i]]), want[i], (what, i))
+
+    run(ia, want_a, False, "eager a")
+    rcols, scols = workloads.plain_small(log_r=10, log_s=14)
+    for rep in range(3):
+        run(ia, want_a, True, "graph a %d" % rep)
+        r, s = _run(ctx, rcols, scols)  # eager work between replays
+        torch.cuda.synchronize()
+        assert len(r) > 0
+    run(ib, want_b, True, "graph b")
+    run(ib, want_b, True, "graph b again")
+    run(ia, want_a, True, "graph a after b")
+    run(ia, want_a, False, "eager a after graphs")
+
+
 def test_budget_admission(ctx):
     rcols, scols = workloads.plain_small(log_r=14, log_s=18)
     dr, ds = workloads.to_device(rcols), workloads.to_device(scols)

@@ -43,6 +43,13 @@ struct vj_ctx {
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
     hipEvent_t ev[EV_COUNT] = {};
+    // Captured pipeline (vj_join_batch with use_graph): the graph is reused
+    // while the launch signature (every kernel argument, grid and copy) is
+    // unchanged; descriptors are re-read from the pinned buffer at each replay.
+    hipStream_t capture_stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    std::vector<uint64_t> graph_sig;
+    uint64_t graph_replays = 0, graph_captures = 0;
 };
 
 namespace {
@@ -367,6 +374,17 @@ PipeArgs make_args(vj_ctx *c, const Plan &pl) {
     return a;
 }
 
+// Writes the descriptors into the pinned staging buffer.
+void stage_descriptors(vj_ctx *c, const Plan &pl) {
+    char *h = static_cast<char *>(c->pinned);
+    memcpy(h + pl.off_jobs, pl.jobs.data(), pl.jobs.size() * sizeof(JobDesc));
+    if (!pl.joins.empty()) memcpy(h + pl.off_joins, pl.joins.data(), pl.joins.size() * sizeof(JoinDesc));
+    if (!pl.colscan.empty()) memcpy(h + pl.off_cs, pl.colscan.data(), pl.colscan.size() * sizeof(ColScanDesc));
+    memcpy(h + pl.off_hb0, pl.hist_block0.data(), pl.hist_block0.size() * 4);
+    memcpy(h + pl.off_pt0, pl.p1_tile0.data(), pl.p1_tile0.size() * 4);
+    memcpy(h + pl.off_pc0, pl.p2_chunk0.data(), pl.p2_chunk0.size() * 4);
+}
+
 Err upload(vj_ctx *c, const Plan &pl, hipStream_t s) {
     char *h = static_cast<char *>(c->pinned);
     memcpy(h + pl.off_jobs, pl.jobs.data(), pl.jobs.size() * sizeof(JobDesc));
@@ -386,10 +404,12 @@ Err upload(vj_ctx *c, const Plan &pl, hipStream_t s) {
             return Err::make(VJ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
     } while (0)
 
-// Launches every stage on `s` (no host synchronisation).
-Err launch_stages(vj_ctx *c, const Plan &pl, const PipeArgs &a, hipStream_t s, bool with_probe) {
-    const bool prof = c->profiling;
-    VJ_HIP_TRY(hipEventRecord(c->ev[EV_START], s));
+// Launches every stage on `s` (no host synchronisation). `events`: record
+// the start / end events (and the per-stage ones when profiling); off while
+// a graph is being captured.
+Err launch_stages(vj_ctx *c, const Plan &pl, const PipeArgs &a, hipStream_t s, bool with_probe, bool events = true) {
+    const bool prof = c->profiling && events;
+    if (events) VJ_HIP_TRY(hipEventRecord(c->ev[EV_START], s));
     VJ_LAUNCH(launch_prepare(a, s));
     if (pl.any_part) VJ_LAUNCH(launch_hist(a, pl.shape, s));
     if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_HIST], s));
@@ -401,8 +421,65 @@ Err launch_stages(vj_ctx *c, const Plan &pl, const PipeArgs &a, hipStream_t s, b
     if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_P2], s));
     if (with_probe && !pl.joins.empty())
         VJ_LAUNCH(launch_probe(a, pl.shape, c->cus * probe_blocks_per_cu(pl.shape), s));
-    VJ_HIP_TRY(hipEventRecord(c->ev[EV_PROBE], s));
+    if (events) VJ_HIP_TRY(hipEventRecord(c->ev[EV_PROBE], s));
     return Err::ok();
+}
+
+// Everything a captured pipeline bakes in: the kernel arguments (arena
+// pointers, totals, hash), which stages run, the probe grid, and the copies.
+std::vector<uint64_t> graph_signature(const vj_ctx *c, const Plan &pl, const PipeArgs &a) {
+    std::vector<uint64_t> sig((sizeof(PipeArgs) + 7) / 8 + 10, 0);
+    memcpy(sig.data(), &a, sizeof(PipeArgs));
+    uint64_t *x = sig.data() + (sizeof(PipeArgs) + 7) / 8;
+    x[0] = (uint64_t)pl.any_part | (uint64_t)pl.any_p2 << 1;
+    x[1] = (uint64_t)pl.shape.hp | (uint64_t)pl.shape.kw << 8 | (uint64_t)pl.shape.nc << 16;
+    x[2] = (uint64_t)c->cus * (uint64_t)probe_blocks_per_cu(pl.shape);
+    x[3] = pl.upload_bytes;
+    x[4] = pl.off_count;
+    x[5] = pl.joins.size();
+    x[6] = reinterpret_cast<uintptr_t>(c->pinned);
+    x[7] = reinterpret_cast<uintptr_t>(c->meta.p);
+    x[8] = pl.colscan.size();
+    x[9] = (uint64_t)c->device;
+    return sig;
+}
+
+void drop_graph(vj_ctx *c) {
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+    c->graph_sig.clear();
+}
+
+// Captures upload -> all stages -> count download on the context's capture
+// stream (capture needs a stream other than the null stream) and instantiates it.
+Err capture_pipeline(vj_ctx *c, const Plan &pl, const PipeArgs &a) {
+    drop_graph(c);
+    if (!c->capture_stream) VJ_HIP_TRY(hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking));
+    hipStream_t cs = c->capture_stream;
+    char *h = static_cast<char *>(c->pinned);
+    VJ_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+    hipError_t he = hipMemcpyAsync(c->meta.p, h, pl.upload_bytes, hipMemcpyHostToDevice, cs);
+    Err e = he == hipSuccess ? launch_stages(c, pl, a, cs, true, false)
+                             : Err::make(VJ_ERR_HIP, std::string("capture upload: ") + hipGetErrorString(he));
+    if (!e.failed()) {
+        he = hipMemcpyAsync(h + pl.off_count, static_cast<char *>(c->meta.p) + pl.off_count, pl.joins.size() * 8,
+                            hipMemcpyDeviceToHost, cs);
+        if (he != hipSuccess) e = Err::make(VJ_ERR_HIP, std::string("capture download: ") + hipGetErrorString(he));
+    }
+    hipGraph_t g = nullptr;
+    he = hipStreamEndCapture(cs, &g);
+    if (!e.failed() && he != hipSuccess) e = Err::make(VJ_ERR_HIP, std::string("end capture: ") + hipGetErrorString(he));
+    if (!e.failed()) {
+        he = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        if (he != hipSuccess) {
+            c->graph = nullptr;
+            e = Err::make(VJ_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(he));
+        }
+    }
+    if (g) (void)hipGraphDestroy(g);
+    if (e.failed()) (void)hipGetLastError();
+    c->graph_captures++;
+    return e;
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -450,18 +527,33 @@ Err set_device(vj_ctx *c) {
 // When `grow` is set, joins whose output overflowed get a fresh buffer of the
 // exact size (hipMalloc, owned by the caller) and only build+probe is rerun.
 Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hipStream_t s,
-              std::vector<uint64_t> &counts, bool grow, uint64_t extra_bytes, int *retries_out) {
+              std::vector<uint64_t> &counts, bool grow, uint64_t extra_bytes, int *retries_out,
+              bool use_graph = false) {
     Plan pl;
     VJ_TRY(build_join_plan(pl, specs, o));
     VJ_TRY(admit(c, pl, extra_bytes));
     VJ_TRY(allocate(c, pl));
-    VJ_TRY(upload(c, pl, s));
     PipeArgs a = make_args(c, pl);
-    VJ_TRY(launch_stages(c, pl, a, s, true));
     char *h = static_cast<char *>(c->pinned);
     unsigned long long *hc = reinterpret_cast<unsigned long long *>(h + pl.off_count);
-    VJ_HIP_TRY(hipMemcpyAsync(hc, static_cast<char *>(c->meta.p) + pl.off_count, pl.joins.size() * 8,
-                              hipMemcpyDeviceToHost, s));
+    if (use_graph) {
+        // the previous use of the pinned buffer has completed (every call ends synchronised)
+        stage_descriptors(c, pl);
+        std::vector<uint64_t> sig = graph_signature(c, pl, a);
+        if (!c->graph || sig != c->graph_sig) {
+            VJ_TRY(capture_pipeline(c, pl, a));
+            c->graph_sig = std::move(sig);
+        }
+        VJ_HIP_TRY(hipEventRecord(c->ev[EV_START], s));
+        VJ_HIP_TRY(hipGraphLaunch(c->graph, s));
+        VJ_HIP_TRY(hipEventRecord(c->ev[EV_PROBE], s));
+        c->graph_replays++;
+    } else {
+        VJ_TRY(upload(c, pl, s));
+        VJ_TRY(launch_stages(c, pl, a, s, true));
+        VJ_HIP_TRY(hipMemcpyAsync(hc, static_cast<char *>(c->meta.p) + pl.off_count, pl.joins.size() * 8,
+                                  hipMemcpyDeviceToHost, s));
+    }
     VJ_HIP_TRY(hipStreamSynchronize(s));
     counts.assign(hc, hc + pl.joins.size());
     int retries = 0;
@@ -538,6 +630,8 @@ vj_status vj_ctx_destroy(vj_ctx *c) {
     if (!c) return VJ_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
+    drop_graph(c);
+    if (c->capture_stream) (void)hipStreamDestroy(c->capture_stream);
     free_all(c);
     if (c->pinned) (void)hipHostFree(c->pinned);
     for (int i = 0; i < EV_COUNT; i++)
@@ -696,7 +790,6 @@ vj_status vj_join_batch(vj_ctx *c, int32_t n_joins, const vj_table *builds, cons
                         const uint64_t *caps, uint64_t *counts, int32_t use_graph, void *stream) {
     if (!c || n_joins <= 0 || !builds || !probes || !out_build || !out_probe || !caps || !counts)
         return VJ_ERR_INVALID_ARGUMENT;
-    (void)use_graph;
     c->last_error.clear();
     vj_join_opts d;
     vj_join_opts_default(&d);
@@ -710,7 +803,7 @@ vj_status vj_join_batch(vj_ctx *c, int32_t n_joins, const vj_table *builds, cons
             return fail(c, Err::make(VJ_ERR_INVALID_ARGUMENT, "output is NULL"));
     }
     std::vector<uint64_t> cnt;
-    e = run_joins(c, specs, opts, (hipStream_t)stream, cnt, false, 0, nullptr);
+    e = run_joins(c, specs, opts, (hipStream_t)stream, cnt, false, 0, nullptr, use_graph != 0);
     if (e.failed()) return fail(c, e);
     bool over = false;
     for (int i = 0; i < n_joins; i++) {

@@ -224,6 +224,85 @@ vj_status vj_join_batch(vj_ctx *ctx, int32_t n_joins, const vj_table *builds, co
                         const vj_join_opts *opts, uint32_t *const *out_build, uint32_t *const *out_probe,
                         const uint64_t *caps, uint64_t *counts, int32_t use_graph, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Resident build sides. No counterpart in the reference, whose partitioned  */
+/* tables live for one join (join_gpu, join_provider.cuh:286-421): a served  */
+/* table is partitioned once and probed by many later requests.             */
+/* ------------------------------------------------------------------------ */
+
+typedef struct vj_built vj_built;
+
+/* Hashes and radix-partitions `build` (planner bits, at least one pass) into
+ * a new device allocation that outlives the call. Synchronous. */
+vj_status vj_build(vj_ctx *ctx, const vj_table *build, const vj_join_opts *opts, vj_built **out, void *stream);
+/* Joins `probe` against a resident build side: only the probe side is hashed
+ * and partitioned (with the table's bits). opts->hash must equal the hash the
+ * table was built with; passes / bits are ignored. Same output rules as
+ * vj_join_into. */
+vj_status vj_probe_into(vj_ctx *ctx, const vj_built *built, const vj_table *probe, const vj_join_opts *opts,
+                        uint32_t *out_build, uint32_t *out_probe, uint64_t capacity, uint64_t *count, void *stream);
+vj_status vj_built_bytes(const vj_built *built, uint64_t *bytes);
+/* Frees the table (hipFree: waits for the device). No join may still use it. */
+vj_status vj_built_free(vj_built *built);
+
+/* ------------------------------------------------------------------------ */
+/* Serving engine: continuous batching of join requests under an HBM budget. */
+/* The reference's nearest analogue is JoinProvider::probe, which runs       */
+/* bucket pairs on host threads and admits them against free device memory  */
+/* (join_provider.cuh:792-911, DeviceConfig :124-220).                       */
+/* ------------------------------------------------------------------------ */
+
+typedef struct vj_server vj_server;
+
+typedef struct vj_request {
+    const vj_table *build; /* build side, or NULL to probe the resident table `table` */
+    uint64_t table;        /* resident table id (from an earlier response's `table`) when build is NULL */
+    const vj_table *probe;
+    uint32_t *out_build, *out_probe; /* caller's device buffers of `capacity` pairs */
+    uint64_t capacity;
+    int32_t keep_build;    /* keep this request's partitioned build side resident (its id is returned) */
+    int32_t reserved;
+} vj_request;
+
+typedef struct vj_response {
+    int32_t status;        /* vj_status of the request */
+    uint32_t batch_size;   /* requests in the launch that ran it */
+    uint64_t count;        /* matches (if > capacity: VJ_ERR_CAPACITY, first `capacity` pairs written) */
+    uint64_t table;        /* id of the kept build side (keep_build), else 0 */
+    uint64_t t_submit_ns, t_launch_ns, t_done_ns; /* steady clock */
+} vj_response;
+
+typedef struct vj_server_stats {
+    uint64_t submitted, completed, failed, batches;
+    uint64_t overlapped_batches; /* launched while another batch was still on the device */
+    uint64_t max_batch;          /* most requests in one launch */
+    uint64_t resident_tables, resident_bytes;
+    uint64_t peak_bytes;         /* most device memory held (arenas + resident tables) */
+    uint64_t budget_waits;       /* times the head request waited for memory */
+} vj_server_stats;
+
+/* Starts the engine on `device` with a worker thread and two launch lanes
+ * (each its own context, arenas and stream), so a batch can be launched
+ * while the previous one runs. hbm_budget caps arenas + resident tables
+ * (0 = none); max_batch caps requests per launch; opts (hash, passes, bits,
+ * one_based) apply to every request. */
+vj_status vj_server_create(int device, uint64_t hbm_budget, const vj_join_opts *opts, int32_t max_batch,
+                           vj_server **out);
+/* Queues a request (its tables and outputs must stay valid until it completes).
+ * Requests are admitted in arrival order as lanes and memory free up; all
+ * admitted requests of a launch run in one pipeline. A request that can
+ * never fit the budget completes with VJ_ERR_BUDGET. */
+vj_status vj_server_submit(vj_server *srv, const vj_request *req, uint64_t *ticket);
+/* Blocks until the request completes, fills *resp and forgets the ticket. */
+vj_status vj_server_wait(vj_server *srv, uint64_t ticket, vj_response *resp);
+/* Non-blocking: *done = 1 and *resp filled (ticket forgotten) if complete. */
+vj_status vj_server_poll(vj_server *srv, uint64_t ticket, vj_response *resp, int32_t *done);
+/* Releases a resident table once no queued or running request uses it. */
+vj_status vj_server_drop_table(vj_server *srv, uint64_t table);
+vj_status vj_server_stats_get(vj_server *srv, vj_server_stats *stats);
+/* Finishes queued work, then frees everything. */
+vj_status vj_server_destroy(vj_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "vj_host.hpp"
 #include "vj_internal.hpp"
 #include "vj_pipeline.hpp"
 
@@ -100,11 +101,6 @@ Err ensure_pinned(vj_ctx *c, size_t bytes) {
 
 // --- planning ----------------------------------------------------------------
 
-struct JoinSpec {
-    const vj_table *r, *s;
-    uint32_t *out_r, *out_s;
-    uint64_t cap;
-};
 
 struct Plan {
     std::vector<JobDesc> jobs;
@@ -119,6 +115,8 @@ struct Plan {
     LaunchShape shape{};
     HashCfg hash{};
     int kb = 0;
+    std::vector<std::pair<uint32_t, vj_built *>> keeps;  // (job, table to fill)
+    uint64_t keep_bytes = 0;
     // meta layout
     size_t off_jobs = 0, off_joins = 0, off_cs = 0, off_hb0 = 0, off_pt0 = 0, off_pc0 = 0, upload_bytes = 0;
     size_t off_jtasks = 0, off_task0 = 0, off_count = 0, off_ctl = 0, meta_bytes = 0;
@@ -169,7 +167,55 @@ Err check_pair(const vj_table *r, const vj_table *s) {
 
 constexpr uint32_t kMaxSegments = 256;  // segments (histogram blocks) per job
 
-void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0) {
+// Bytes of a resident table of n rows.
+size_t built_bytes(uint64_t n, int nc, int kw, uint32_t b1, uint32_t b2) {
+    return align_up((size_t)kw * n) * nc + align_up((size_t)4 * n) + align_up((size_t)4 * ((1u << (b1 + b2)) + 1));
+}
+
+Err alloc_built(vj_built *b) {
+    b->bytes = built_bytes(b->n, b->n_cols, b->key_width, b->b1, b->b2);
+    if (hipMalloc(&b->mem, b->bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        b->mem = nullptr;
+        return Err::make(VJ_ERR_OUT_OF_MEMORY, "hipMalloc of a resident table failed");
+    }
+    char *p = static_cast<char *>(b->mem);
+    const size_t col = align_up((size_t)b->key_width * b->n);
+    for (int c = 0; c < 4; c++) b->cols[c] = c < b->n_cols ? p + c * col : nullptr;
+    b->ids = reinterpret_cast<uint32_t *>(p + b->n_cols * col);
+    b->base = reinterpret_cast<uint32_t *>(p + b->n_cols * col + align_up((size_t)4 * b->n));
+    return Err::ok();
+}
+
+void free_built(vj_built *b) {
+    if (b->mem) (void)hipFree(b->mem);
+    b->mem = nullptr;
+}
+
+// A resident build side as an already partitioned job: no hist / pass work.
+uint32_t add_resident_job(Plan &pl, const vj_built *b) {
+    JobDesc jd;
+    memset(&jd, 0, sizeof(jd));
+    jd.in.n = b->n;
+    jd.n = (uint32_t)b->n;
+    jd.b1 = b->b1;
+    jd.b2 = b->b2;
+    for (int c = 0; c < 4; c++) jd.fin_cols[c] = b->cols[c];
+    jd.fin_ids = b->ids;
+    jd.res_base = b->base;
+    jd.base_off = (uint32_t)pl.base_words;
+    pl.base_words += (1u << (b->b1 + b->b2)) + 1;
+    pl.any_part = true;
+    pl.hist_block0.push_back(pl.total_hist_blocks);
+    pl.p1_tile0.push_back(pl.total_p1_tiles);
+    pl.p2_chunk0.push_back(pl.total_p2_chunks);
+    pl.fin_off.push_back(SIZE_MAX);
+    pl.mid_off.push_back(SIZE_MAX);
+    pl.jobs.push_back(jd);
+    return (uint32_t)pl.jobs.size() - 1;
+}
+
+void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0, bool fin_external = false) {
     JobDesc jd;
     memset(&jd, 0, sizeof(jd));
     jd.in = to_view(t);
@@ -217,8 +263,10 @@ void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0
     size_t fin = SIZE_MAX, mid = SIZE_MAX;
     if (jd.b1) {
         const size_t one = align_up((size_t)t->key_width * jd.n) * t->n_cols + align_up((size_t)4 * jd.n);
-        fin = pl.tuples_bytes;
-        pl.tuples_bytes += one;
+        if (!fin_external) {
+            fin = pl.tuples_bytes;
+            pl.tuples_bytes += one;
+        }
         if (jd.b2) {
             mid = pl.tuples_bytes;
             pl.tuples_bytes += one;
@@ -259,8 +307,13 @@ void finish_layout(Plan &pl) {
     pl.meta_bytes = o;
 }
 
+bool same_hash(const vj_hash_opts &a, const vj_hash_opts &b) {
+    return a.kind == b.kind && a.hash_bits == b.hash_bits && a.chunk_bits == b.chunk_bits && a.word_mode == b.word_mode;
+}
+
 Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_opts *o) {
-    const vj_table *t0 = specs[0].r;
+    const vj_table *t0 = specs[0].s;
+    VJ_TRY(check_table(t0));
     VJ_TRY(make_hash_cfg(&o->hash, t0, &pl.hash));
     pl.kb = t0->key_width * t0->n_cols;
     pl.shape.hp = hash_path(pl.hash);
@@ -268,16 +321,47 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
     pl.shape.nc = t0->n_cols;
     for (size_t i = 0; i < specs.size(); i++) {
         const JoinSpec &js = specs[i];
-        VJ_TRY(check_pair(js.r, js.s));
-        if (js.r->n_cols != t0->n_cols || js.r->key_width != t0->key_width ||
-            (js.r->key_signed != 0) != (t0->key_signed != 0))
-            return Err::make(VJ_ERR_INVALID_ARGUMENT, "all joins of a batch must share the key columns' shape");
         int P, b1, b2;
-        VJ_TRY(plan_bits(js.r->n_rows, pl.kb, o, &P, &b1, &b2));
         JoinDesc jd;
         memset(&jd, 0, sizeof(jd));
-        jd.job_r = (uint32_t)pl.jobs.size();
-        add_job(pl, js.r, P, b1, b2, 0);
+        if (js.res) {
+            VJ_TRY(check_table(js.s));
+            const vj_built *b = js.res;
+            if (b->n_cols != js.s->n_cols || b->key_width != js.s->key_width ||
+                (b->key_signed != 0) != (js.s->key_signed != 0))
+                return Err::make(VJ_ERR_INVALID_ARGUMENT, "probe table and resident table have different key columns");
+            if (!same_hash(b->hash, o->hash))
+                return Err::make(VJ_ERR_INVALID_ARGUMENT, "the resident table was partitioned with another hash");
+            P = b->b2 ? 2 : 1;
+            b1 = (int)b->b1;
+            b2 = (int)b->b2;
+            jd.job_r = add_resident_job(pl, b);
+        } else {
+            VJ_TRY(check_pair(js.r, js.s));
+            VJ_TRY(plan_bits(js.r->n_rows, pl.kb, o, &P, &b1, &b2));
+            if (js.keep && P == 0) {  // a kept table is always partitioned
+                P = 1;
+                b1 = 1;
+                b2 = 0;
+            }
+            jd.job_r = (uint32_t)pl.jobs.size();
+            add_job(pl, js.r, P, b1, b2, 0, js.keep != nullptr);
+            if (js.keep) {
+                vj_built *k = js.keep;
+                k->n = js.r->n_rows;
+                k->n_cols = js.r->n_cols;
+                k->key_width = js.r->key_width;
+                k->key_signed = js.r->key_signed != 0;
+                k->hash = o->hash;
+                k->b1 = (uint32_t)b1;
+                k->b2 = (uint32_t)(P >= 2 ? b2 : 0);
+                pl.keeps.push_back({jd.job_r, k});
+                pl.keep_bytes += built_bytes(k->n, k->n_cols, k->key_width, k->b1, k->b2);
+            }
+        }
+        if (js.s->n_cols != t0->n_cols || js.s->key_width != t0->key_width ||
+            (js.s->key_signed != 0) != (t0->key_signed != 0))
+            return Err::make(VJ_ERR_INVALID_ARGUMENT, "all joins of a batch must share the key columns' shape");
         jd.job_s = (uint32_t)pl.jobs.size();
         add_job(pl, js.s, P, b1, b2, 0);
         jd.out_r = js.out_r;
@@ -341,6 +425,18 @@ Err allocate(vj_ctx *c, Plan &pl) {
             for (int cc = 0; cc < 4; cc++) jd.mid_cols[cc] = cc < pl.shape.nc ? p + cc * col : nullptr;
             jd.mid_ids = reinterpret_cast<uint32_t *>(p + pl.shape.nc * col);
         }
+    }
+    // kept build sides: their own allocation receives the last pass and the bases
+    for (auto &kp : pl.keeps) {
+        vj_built *b = kp.second;
+        if (!b->mem) {
+            b->device = c->device;
+            VJ_TRY(alloc_built(b));
+        }
+        JobDesc &jd = pl.jobs[kp.first];
+        for (int cc = 0; cc < 4; cc++) jd.fin_cols[cc] = b->cols[cc];
+        jd.fin_ids = b->ids;
+        jd.keep_base = b->base;
     }
     return Err::ok();
 }
@@ -494,7 +590,7 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 void fill_summary(vj_ctx *c, const Plan &pl, const JoinSpec &js, uint64_t matches, int retries) {
     vj_join_summary &s = c->summary;
     memset(&s, 0, sizeof(s));
-    s.build_rows = js.r->n_rows;
+    s.build_rows = js.res ? js.res->n : js.r->n_rows;
     s.probe_rows = js.s->n_rows;
     s.matches = matches;
     const JobDesc &jr = pl.jobs[0];
@@ -531,7 +627,7 @@ Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hi
               bool use_graph = false) {
     Plan pl;
     VJ_TRY(build_join_plan(pl, specs, o));
-    VJ_TRY(admit(c, pl, extra_bytes));
+    VJ_TRY(admit(c, pl, extra_bytes + pl.keep_bytes));
     VJ_TRY(allocate(c, pl));
     PipeArgs a = make_args(c, pl);
     char *h = static_cast<char *>(c->pinned);
@@ -596,6 +692,49 @@ Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hi
 }
 
 }  // namespace
+
+namespace vj {
+
+Err pipeline_bytes(const std::vector<JoinSpec> &specs, const vj_join_opts *o, uint64_t *bytes) {
+    Plan pl;
+    VJ_TRY(build_join_plan(pl, specs, o));
+    const uint64_t scratch = plan_scratch_bytes(pl);
+    *bytes = scratch + scratch / 8 + pl.keep_bytes;
+    return Err::ok();
+}
+
+Err pipeline_launch(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hipStream_t s, uint64_t budget,
+                    size_t *count_off) {
+    VJ_TRY(set_device(c));
+    Plan pl;
+    VJ_TRY(build_join_plan(pl, specs, o));
+    c->budget = budget;
+    VJ_TRY(admit(c, pl, pl.keep_bytes));
+    VJ_TRY(allocate(c, pl));
+    VJ_TRY(upload(c, pl, s));
+    PipeArgs a = make_args(c, pl);
+    VJ_TRY(launch_stages(c, pl, a, s, true));
+    char *h = static_cast<char *>(c->pinned);
+    VJ_HIP_TRY(hipMemcpyAsync(h + pl.off_count, static_cast<char *>(c->meta.p) + pl.off_count, pl.joins.size() * 8,
+                              hipMemcpyDeviceToHost, s));
+    *count_off = pl.off_count;
+    return Err::ok();
+}
+
+void pipeline_counts(const vj_ctx *c, size_t count_off, size_t n, std::vector<uint64_t> &counts) {
+    const unsigned long long *hc =
+        reinterpret_cast<const unsigned long long *>(static_cast<const char *>(c->pinned) + count_off);
+    counts.assign(hc, hc + n);
+}
+
+uint64_t ctx_bytes(const vj_ctx *c) { return held_bytes(c); }
+void ctx_release(vj_ctx *c) {
+    drop_graph(c);
+    free_all(c);
+}
+void built_release(vj_built *b) { free_built(b); }
+
+}  // namespace vj
 
 extern "C" {
 
@@ -811,6 +950,106 @@ vj_status vj_join_batch(vj_ctx *c, int32_t n_joins, const vj_table *builds, cons
         over |= cnt[i] > caps[i];
     }
     if (over) return fail(c, Err::make(VJ_ERR_CAPACITY, "an output buffer of the batch is too small"));
+    return VJ_OK;
+}
+
+vj_status vj_build(vj_ctx *c, const vj_table *build, const vj_join_opts *opts, vj_built **out, void *stream) {
+    if (!c || !out) return VJ_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    c->last_error.clear();
+    vj_join_opts d;
+    vj_join_opts_default(&d);
+    if (!opts) opts = &d;
+    Err e = check_table(build);
+    if (e.failed()) return fail(c, e);
+    e = set_device(c);
+    if (e.failed()) return fail(c, e);
+    hipStream_t s = (hipStream_t)stream;
+    vj_built *b = new vj_built();
+    Plan pl;
+    e = make_hash_cfg(&opts->hash, build, &pl.hash);
+    int P = 0, b1 = 0, b2 = 0;
+    if (!e.failed()) {
+        pl.kb = build->key_width * build->n_cols;
+        pl.shape.hp = hash_path(pl.hash);
+        pl.shape.kw = build->key_width;
+        pl.shape.nc = build->n_cols;
+        e = plan_bits(build->n_rows, pl.kb, opts, &P, &b1, &b2);
+    }
+    if (!e.failed()) {
+        if (P == 0) {
+            P = 1;
+            b1 = 1;
+            b2 = 0;
+        }
+        add_job(pl, build, P, b1, b2, 0, true);
+        b->n = build->n_rows;
+        b->n_cols = build->n_cols;
+        b->key_width = build->key_width;
+        b->key_signed = build->key_signed != 0;
+        b->hash = opts->hash;
+        b->b1 = (uint32_t)b1;
+        b->b2 = (uint32_t)(P >= 2 ? b2 : 0);
+        b->device = c->device;
+        pl.keeps.push_back({0u, b});
+        pl.keep_bytes = built_bytes(b->n, b->n_cols, b->key_width, b->b1, b->b2);
+        finish_layout(pl);
+        e = admit(c, pl, pl.keep_bytes);
+    }
+    if (!e.failed()) e = allocate(c, pl);
+    if (!e.failed()) e = upload(c, pl, s);
+    if (!e.failed()) {
+        PipeArgs a = make_args(c, pl);
+        e = launch_stages(c, pl, a, s, false);
+    }
+    if (!e.failed()) {
+        hipError_t he = hipStreamSynchronize(s);
+        if (he != hipSuccess) e = Err::make(VJ_ERR_HIP, hipGetErrorString(he));
+    }
+    if (e.failed()) {
+        free_built(b);
+        delete b;
+        return fail(c, e);
+    }
+    *out = b;
+    return VJ_OK;
+}
+
+vj_status vj_probe_into(vj_ctx *c, const vj_built *built, const vj_table *probe, const vj_join_opts *opts,
+                        uint32_t *out_build, uint32_t *out_probe, uint64_t capacity, uint64_t *count, void *stream) {
+    if (!c || !built || !count) return VJ_ERR_INVALID_ARGUMENT;
+    c->last_error.clear();
+    *count = 0;
+    vj_join_opts d;
+    vj_join_opts_default(&d);
+    if (!opts) opts = &d;
+    if (capacity && (!out_build || !out_probe)) return fail(c, Err::make(VJ_ERR_INVALID_ARGUMENT, "output is NULL"));
+    if (built->device != c->device) return fail(c, Err::make(VJ_ERR_INVALID_ARGUMENT, "table is on another device"));
+    Err e = set_device(c);
+    if (e.failed()) return fail(c, e);
+    std::vector<JoinSpec> specs{{nullptr, probe, out_build, out_probe, capacity}};
+    specs[0].res = built;
+    std::vector<uint64_t> counts;
+    e = run_joins(c, specs, opts, (hipStream_t)stream, counts, false, 0, nullptr);
+    if (e.failed()) return fail(c, e);
+    *count = counts[0];
+    if (counts[0] > capacity)
+        return fail(c, Err::make(VJ_ERR_CAPACITY, "output holds " + std::to_string(capacity) + " pairs, join has " +
+                                                      std::to_string(counts[0])));
+    return VJ_OK;
+}
+
+vj_status vj_built_bytes(const vj_built *b, uint64_t *bytes) {
+    if (!b || !bytes) return VJ_ERR_INVALID_ARGUMENT;
+    *bytes = b->bytes;
+    return VJ_OK;
+}
+
+vj_status vj_built_free(vj_built *b) {
+    if (!b) return VJ_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(b->device);
+    free_built(b);
+    delete b;
     return VJ_OK;
 }
 

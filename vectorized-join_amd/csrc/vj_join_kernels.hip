@@ -247,12 +247,20 @@ __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
         return;
     }
     const uint32_t nb = 1u << (job.b1 + job.b2);
-    if (job.n == 0) {  // an empty job has no count rows
-        for (uint32_t q = threadIdx.x; q <= nb; q += blockDim.x) base[q] = 0;
+    if (job.res_base) {  // resident table: its bases were computed when it was built
+        for (uint32_t q = threadIdx.x; q <= nb; q += blockDim.x) base[q] = job.res_base[q];
         return;
     }
-    const uint32_t *tot = a.counts + (job.b2 ? job.m2_off : job.m1_off) + (uint64_t)job.nseg * nb;
-    block_scan_array<uint32_t>(tot, base, nb, sh32);
+    if (job.n == 0) {  // an empty job has no count rows
+        for (uint32_t q = threadIdx.x; q <= nb; q += blockDim.x) base[q] = 0;
+    } else {
+        const uint32_t *tot = a.counts + (job.b2 ? job.m2_off : job.m1_off) + (uint64_t)job.nseg * nb;
+        block_scan_array<uint32_t>(tot, base, nb, sh32);
+    }
+    if (job.keep_base) {
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q <= nb; q += blockDim.x) job.keep_base[q] = base[q];
+    }
 }
 
 // One block of 1024 threads per join: build/probe tasks = R chunks x S chunks

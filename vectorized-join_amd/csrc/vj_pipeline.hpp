@@ -63,6 +63,12 @@ struct JobDesc {
     uint64_t tilepre_off;     // u32 offset of tilepre [p1_tiles][2^b1] in the count arena
     uint64_t m1_off;          // u32 offset of m1 [nseg + 1][2^b1]
     uint64_t m2_off;          // u32 offset of m2 [nseg + 1][2^(b1+b2)] (2-pass jobs)
+    // Resident build sides. res_base != null: the job is already partitioned
+    // (fin_* point at a resident table, res_base at its 2^(b1+b2)+1 bases) and
+    // no hist / pass work is scheduled for it. keep_base != null: the bases
+    // are also written there (the job's fin_* point at a table being kept).
+    const uint32_t *res_base;
+    uint32_t *keep_base;
 };
 
 struct JoinDesc {

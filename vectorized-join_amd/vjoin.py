@@ -52,6 +52,24 @@ class Pairs(ctypes.Structure):
     _fields_ = [("count", ctypes.c_uint64), ("build_rowids", ctypes.c_void_p), ("probe_rowids", ctypes.c_void_p)]
 
 
+class Request(ctypes.Structure):
+    _fields_ = [("build", ctypes.c_void_p), ("table", ctypes.c_uint64), ("probe", ctypes.c_void_p),
+                ("out_build", ctypes.c_void_p), ("out_probe", ctypes.c_void_p), ("capacity", ctypes.c_uint64),
+                ("keep_build", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class Response(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("batch_size", ctypes.c_uint32), ("count", ctypes.c_uint64),
+                ("table", ctypes.c_uint64), ("t_submit_ns", ctypes.c_uint64), ("t_launch_ns", ctypes.c_uint64),
+                ("t_done_ns", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "failed", "batches", "overlapped_batches",
+                                                "max_batch", "resident_tables", "resident_bytes", "peak_bytes",
+                                                "budget_waits")]
+
+
 class JoinSummary(ctypes.Structure):
     _fields_ = [("build_rows", ctypes.c_uint64), ("probe_rows", ctypes.c_uint64), ("matches", ctypes.c_uint64),
                 ("passes", ctypes.c_int32), ("bits1", ctypes.c_int32), ("bits2", ctypes.c_int32),
@@ -95,6 +113,17 @@ def lib():
             "vj_partition": ([vp, P(Table), P(HashOpts), i32, i32, P(Table), vp, vp, vp], i32),
             "vj_join_batch": ([vp, i32, P(Table), P(Table), P(JoinOpts), P(vp), P(vp), P(u64), P(u64), i32, vp],
                               i32),
+            "vj_build": ([vp, P(Table), P(JoinOpts), P(vp), vp], i32),
+            "vj_probe_into": ([vp, vp, P(Table), P(JoinOpts), vp, vp, u64, P(u64), vp], i32),
+            "vj_built_bytes": ([vp, P(u64)], i32),
+            "vj_built_free": ([vp], i32),
+            "vj_server_create": ([ctypes.c_int, u64, P(JoinOpts), i32, P(vp)], i32),
+            "vj_server_submit": ([vp, P(Request), P(u64)], i32),
+            "vj_server_wait": ([vp, u64, P(Response)], i32),
+            "vj_server_poll": ([vp, u64, P(Response), P(i32)], i32),
+            "vj_server_drop_table": ([vp, u64], i32),
+            "vj_server_stats_get": ([vp, P(ServerStats)], i32),
+            "vj_server_destroy": ([vp], i32),
         }
         for name, (args, res) in list(sigs.items()) + list(optional.items()):
             if not hasattr(L, name):
@@ -237,6 +266,24 @@ class Context:
                                   ctypes.byref(out_tab), ctypes.c_void_p(hist.data_ptr()),
                                   ctypes.c_void_p(offsets.data_ptr()), stream_ptr())
 
+    def build(self, table, opts):
+        """vj_build: a resident, partitioned build side (Built)."""
+        h = ctypes.c_void_p()
+        check(lib().vj_build(self.ptr, ctypes.byref(table), ctypes.byref(opts), ctypes.byref(h), stream_ptr()),
+              "vj_build", self.ptr)
+        return Built(h)
+
+    def probe_into(self, built, probe, opts, out_r, out_s, capacity=None):
+        """vj_probe_into; returns (status, count). Does not raise on VJ_ERR_CAPACITY."""
+        cnt = ctypes.c_uint64()
+        cap = int(out_r.numel()) if capacity is None else int(capacity)
+        rc = lib().vj_probe_into(self.ptr, built.ptr, ctypes.byref(probe), ctypes.byref(opts),
+                                 ctypes.c_void_p(out_r.data_ptr()), ctypes.c_void_p(out_s.data_ptr()), cap,
+                                 ctypes.byref(cnt), stream_ptr())
+        if rc not in (VJ_OK, STATUS_CODE["VJ_ERR_CAPACITY"]):
+            check(rc, "vj_probe_into", self.ptr)
+        return rc, cnt.value
+
     def join_batch(self, builds, probes, opts, outs_r, outs_s, caps, use_graph=False):
         n = len(builds)
         B = (Table * n)(*builds)
@@ -248,6 +295,84 @@ class Context:
         rc = lib().vj_join_batch(self.ptr, n, B, S, ctypes.byref(opts), R_ptrs, S_ptrs, C, counts,
                                  1 if use_graph else 0, stream_ptr())
         return rc, [counts[i] for i in range(n)]
+
+
+class Built:
+    """A resident build side (vj_built)."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def nbytes(self):
+        b = ctypes.c_uint64()
+        check(lib().vj_built_bytes(self.ptr, ctypes.byref(b)), "vj_built_bytes")
+        return b.value
+
+    def free(self):
+        if self.ptr:
+            lib().vj_built_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving engine (vj_server): submit requests, wait for responses.
+    Tables and output tensors of a request must stay alive until it completes;
+    the server keeps references to them until then."""
+
+    def __init__(self, device=0, budget=0, opts=None, max_batch=64):
+        self.ptr = ctypes.c_void_p()
+        o = opts if opts is not None else join_opts()
+        check(lib().vj_server_create(device, int(budget), ctypes.byref(o), int(max_batch), ctypes.byref(self.ptr)),
+              "vj_server_create")
+        self._live = {}
+
+    def submit(self, probe, out_r, out_s, build=None, table=0, keep_build=False, capacity=None):
+        req = Request()
+        req.build = ctypes.cast(ctypes.pointer(build), ctypes.c_void_p) if build is not None else None
+        req.table = int(table)
+        req.probe = ctypes.cast(ctypes.pointer(probe), ctypes.c_void_p)
+        req.out_build = out_r.data_ptr()
+        req.out_probe = out_s.data_ptr()
+        req.capacity = int(out_r.numel()) if capacity is None else int(capacity)
+        req.keep_build = 1 if keep_build else 0
+        t = ctypes.c_uint64()
+        check(lib().vj_server_submit(self.ptr, ctypes.byref(req), ctypes.byref(t)), "vj_server_submit")
+        self._live[t.value] = (build, probe, out_r, out_s)
+        return t.value
+
+    def wait(self, ticket):
+        r = Response()
+        check(lib().vj_server_wait(self.ptr, int(ticket), ctypes.byref(r)), "vj_server_wait")
+        self._live.pop(ticket, None)
+        return {"status": r.status, "status_name": STATUS.get(r.status, str(r.status)), "count": r.count,
+                "table": r.table, "batch_size": r.batch_size, "t_submit_ns": r.t_submit_ns,
+                "t_launch_ns": r.t_launch_ns, "t_done_ns": r.t_done_ns}
+
+    def drop_table(self, table):
+        check(lib().vj_server_drop_table(self.ptr, int(table)), "vj_server_drop_table")
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().vj_server_stats_get(self.ptr, ctypes.byref(st)), "vj_server_stats_get")
+        return {n: getattr(st, n) for n, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.ptr:
+            lib().vj_server_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class _DevArray:

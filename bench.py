@@ -244,6 +244,148 @@ def batched_leg(ctx, n_joins, reps, seed=None):
             "same_counts_and_checksums": bool(same)}
 
 
+def serving_leg(n_requests, rates, budget_gb=64, n_pool=64, n_tables=8, n_reprobes=16, seed=7):
+    """Poisson arrivals of join requests: 70 % fresh joins with sizes as the
+    batched workload, 30 % re-probes of one of `n_tables` resident build sides
+    with a fresh probe batch of 2^U(18,24) rows. Served by vj_server (HBM
+    budget, continuous batching over two lanes) and, at the same arrival
+    times, by a plain loop that runs one join per launch in arrival order.
+    Latency = completion - arrival. Every request's pairs must agree (count and
+    checksum) between the two."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    import vjoin
+    import workloads
+    rng = np.random.default_rng(seed)
+    pool = []
+    for rcols, scols in workloads.batched(n_joins=n_pool, seed=seed):
+        dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+        # the tensors stay referenced: vjoin.table only holds their pointers
+        pool.append((vjoin.table(dr), vjoin.table(ds), len(scols[0]), (dr, ds)))
+    tables = []
+    reprobes = []
+    for t in range(n_tables):
+        rcols, _ = workloads.batched(n_joins=1, log_min=16, log_max=20, seed=seed * 1000 + t)[0]
+        tables.append((rcols, workloads.to_device(rcols)))
+    for k in range(n_reprobes):
+        t = k % n_tables
+        keys = tables[t][0][0]
+        m = int(2.0 ** rng.uniform(18, 24))
+        s = np.where(rng.random(m) < 0.5, keys[rng.integers(0, len(keys), size=m)],
+                     rng.integers(len(keys) * 4, 1 << 30, size=m).astype(keys.dtype))
+        ds = workloads.to_device([s.astype(keys.dtype)])
+        reprobes.append((t, vjoin.table(ds), m, ds))
+    opts = vjoin.join_opts("fnv", 64, 64)
+    results = {"workload": "%d requests: 70%% fresh int32 joins (build 2^U(10,20), probe U(4,16)x), 30%% re-probes "
+                           "of %d resident tables with 2^U(18,24)-row probes; Poisson arrivals" % (n_requests, n_tables),
+               "budget_GB": budget_gb, "rates": []}
+
+    # resident tables: the server keeps them from keep_build requests; the plain loop builds them with vj_build
+    srv = vjoin.Server(0, budget=int(budget_gb * (1 << 30)), opts=opts, max_batch=64)
+    ctx = vjoin.Context(0)
+    tid, built = [], []
+    for rcols, dr in tables:
+        probe = vjoin.table(workloads.to_device([rcols[0][:1024]]))
+        o = (torch.empty(1024, dtype=torch.int32, device="cuda"), torch.empty(1024, dtype=torch.int32, device="cuda"))
+        r = srv.wait(srv.submit(probe, o[0], o[1], build=vjoin.table(dr), keep_build=True))
+        assert r["status"] == 0 and r["table"] > 0, r
+        tid.append(r["table"])
+        built.append(ctx.build(vjoin.table(dr), opts))
+
+    for lam in rates:
+        arr = np.cumsum(rng.exponential(1.0 / lam, size=n_requests))
+        kinds = []
+        for i in range(n_requests):
+            if rng.random() < 0.3:
+                kinds.append(("re", int(rng.integers(0, n_reprobes))))
+            else:
+                kinds.append(("fresh", int(rng.integers(0, n_pool))))
+        outs = []
+        for kind, j in kinds:
+            n = pool[j][2] if kind == "fresh" else reprobes[j][2]
+            outs.append((torch.empty(n, dtype=torch.int32, device="cuda"),
+                         torch.empty(n, dtype=torch.int32, device="cuda")))
+        torch.cuda.synchronize()
+
+        # served: a submitter thread releases requests at their arrival times
+        tickets = [None] * n_requests
+        t0_ns = [0]
+
+        def submitter():
+            t0 = time.perf_counter()
+            t0_ns[0] = time.monotonic_ns()
+            for i, (kind, j) in enumerate(kinds):
+                while time.perf_counter() - t0 < arr[i]:
+                    pass
+                o = outs[i]
+                if kind == "fresh":
+                    tickets[i] = srv.submit(pool[j][1], o[0], o[1], build=pool[j][0])
+                else:
+                    t, probe, _, _ = reprobes[j]
+                    tickets[i] = srv.submit(probe, o[0], o[1], table=tid[t])
+
+        st0 = srv.stats()
+        th = threading.Thread(target=submitter)
+        th.start()
+        resp = []
+        for i in range(n_requests):
+            while tickets[i] is None:
+                time.sleep(0)
+            resp.append(srv.wait(tickets[i]))
+        th.join()
+        st1 = srv.stats()
+        assert all(r["status"] == 0 for r in resp), [r for r in resp if r["status"]][:3]
+        lat = np.array([(r["t_done_ns"] - r["t_submit_ns"]) * 1e-6 for r in resp])
+        span = (max(r["t_done_ns"] for r in resp) - min(r["t_submit_ns"] for r in resp)) * 1e-9
+        served_sum = _pair_checksums([o[0] for o in outs], [o[1] for o in outs], [r["count"] for r in resp])
+        served_cnt = [r["count"] for r in resp]
+        for o in outs:
+            o[0].fill_(-1)
+            o[1].fill_(-1)
+        torch.cuda.synchronize()
+
+        # plain: one join per launch, in arrival order, never before its arrival
+        plain_lat, plain_cnt = [], []
+        t0 = time.perf_counter()
+        for i, (kind, j) in enumerate(kinds):
+            while time.perf_counter() - t0 < arr[i]:
+                pass
+            o = outs[i]
+            if kind == "fresh":
+                rc, cnt = ctx.join_into(pool[j][0], pool[j][1], opts, o[0], o[1])
+            else:
+                t, probe, _, _ = reprobes[j]
+                rc, cnt = ctx.probe_into(built[t], probe, opts, o[0], o[1])
+            assert rc == 0
+            plain_lat.append((time.perf_counter() - t0 - arr[i]) * 1e3)
+            plain_cnt.append(cnt)
+        plain_span = time.perf_counter() - t0 - arr[0]
+        plain_sum = _pair_checksums([o[0] for o in outs], [o[1] for o in outs], plain_cnt)
+        plain_lat = np.array(plain_lat)
+        same = served_cnt == plain_cnt and served_sum == plain_sum
+        results["rates"].append({
+            "arrival_rate_per_s": lam,
+            "served": {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                       "joins_per_s": n_requests / span, "batches": st1["batches"] - st0["batches"],
+                       "overlapped_batches": st1["overlapped_batches"] - st0["overlapped_batches"]},
+            "plain": {"p50_ms": float(np.percentile(plain_lat, 50)), "p99_ms": float(np.percentile(plain_lat, 99)),
+                      "joins_per_s": n_requests / plain_span},
+            "same_counts_and_checksums": bool(same)})
+        del outs
+        torch.cuda.empty_cache()
+    st = srv.stats()
+    results["server_stats"] = {k: st[k] for k in ("peak_bytes", "resident_tables", "resident_bytes", "max_batch",
+                                                  "budget_waits")}
+    srv.close()
+    for b in built:
+        b.free()
+    ctx.close()
+    return results
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -258,6 +400,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--batched-joins", type=int, default=256)
     ap.add_argument("--batched-reps", type=int, default=3)
+    ap.add_argument("--no-serving", action="store_true", help="skip the served-vs-plain request stream")
+    ap.add_argument("--serving-requests", type=int, default=512)
+    ap.add_argument("--serving-rates", type=float, nargs="+", default=[2000.0, 8000.0, 32000.0])
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device three times (counter calibration)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -377,6 +522,13 @@ def main():
         torch.cuda.empty_cache()
         print("bench: batched leg (%d joins)" % args.batched_joins, file=sys.stderr, flush=True)
         out["batched"] = batched_leg(ctx, args.batched_joins, args.batched_reps)
+    if not args.no_serving:
+        if args.no_batched:
+            del dr, ds, build, probe, out_r, out_s
+        torch.cuda.empty_cache()
+        print("bench: serving leg (%d requests at %s /s)" % (args.serving_requests, args.serving_rates),
+              file=sys.stderr, flush=True)
+        out["serving"] = serving_leg(args.serving_requests, args.serving_rates)
     if not args.no_cpu:
         print("bench: cpu baseline (OpenMP oracle)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_log_r, args.cpu_log_s, summ["passes"], summ["bits1"],

@@ -110,6 +110,8 @@ struct Plan {
     std::vector<size_t> fin_off, mid_off;  // byte offsets in the tuples arena (SIZE_MAX: none)
     size_t tuples_bytes = 0;
     uint64_t counts_words = 0, base_words = 0, ptask_words = 0;
+    uint64_t taskmap_cap = 0;  // entries of the task map (after the task prefixes in the ptask arena)
+    uint32_t max_nparts = 0;
     uint32_t max_hist_bins = 1, total_hist_blocks = 0, total_p1_tiles = 0, total_p2_chunks = 0;
     bool any_part = false, any_p2 = false;
     LaunchShape shape{};
@@ -372,6 +374,8 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
         jd.nparts = 1u << (P >= 1 ? b1 : 0) << (P >= 2 ? b2 : 0);
         jd.ptask_off = (uint32_t)pl.ptask_words;
         pl.ptask_words += jd.nparts + 1;
+        pl.taskmap_cap += 2ull * jd.nparts;
+        pl.max_nparts = std::max(pl.max_nparts, jd.nparts);
         jd.slot_shift = (P >= 1 ? b1 : 0) + (P >= 2 ? b2 : 0);
         jd.one_based = o->one_based ? 1 : 0;
         pl.joins.push_back(jd);
@@ -382,7 +386,7 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
 
 uint64_t plan_scratch_bytes(const Plan &pl) {
     return align_up(pl.tuples_bytes) + align_up(pl.counts_words * 4) + align_up(pl.base_words * 4) +
-           align_up(pl.ptask_words * 8) + align_up(pl.meta_bytes);
+           align_up((pl.ptask_words + pl.taskmap_cap) * 8) + align_up(pl.meta_bytes);
 }
 
 // Admission (DeviceConfig::is_out_of_memory, join_provider.cuh:148-150): the
@@ -397,7 +401,7 @@ Err admit(vj_ctx *c, const Plan &pl, uint64_t extra) {
     const uint64_t keep = std::max<uint64_t>(c->tuples.bytes, pl.tuples_bytes) +
                           std::max<uint64_t>(c->counts.bytes, pl.counts_words * 4) +
                           std::max<uint64_t>(c->base.bytes, pl.base_words * 4) +
-                          std::max<uint64_t>(c->ptask.bytes, pl.ptask_words * 8) +
+                          std::max<uint64_t>(c->ptask.bytes, (pl.ptask_words + pl.taskmap_cap) * 8) +
                           std::max<uint64_t>(c->meta.bytes, pl.meta_bytes);
     if (keep + extra > c->budget) free_all(c);
     return Err::ok();
@@ -407,7 +411,7 @@ Err allocate(vj_ctx *c, Plan &pl) {
     VJ_TRY(ensure(c->tuples, std::max<size_t>(pl.tuples_bytes, kAlign)));
     VJ_TRY(ensure(c->counts, std::max<size_t>(pl.counts_words * 4, kAlign)));
     VJ_TRY(ensure(c->base, std::max<size_t>(pl.base_words * 4, kAlign)));
-    VJ_TRY(ensure(c->ptask, std::max<size_t>(pl.ptask_words * 8, kAlign)));
+    VJ_TRY(ensure(c->ptask, std::max<size_t>((pl.ptask_words + pl.taskmap_cap) * 8, kAlign)));
     VJ_TRY(ensure(c->meta, pl.meta_bytes));
     VJ_TRY(ensure_pinned(c, pl.meta_bytes));
     // tuple buffer pointers
@@ -458,6 +462,9 @@ PipeArgs make_args(vj_ctx *c, const Plan &pl) {
     a.counts = static_cast<uint32_t *>(c->counts.p);
     a.base = static_cast<uint32_t *>(c->base.p);
     a.ptask = static_cast<unsigned long long *>(c->ptask.p);
+    a.taskmap = a.ptask + pl.ptask_words;
+    a.taskmap_cap = pl.taskmap_cap;
+    a.max_nparts = pl.max_nparts;
     a.n_jobs = (uint32_t)pl.jobs.size();
     a.n_joins = (uint32_t)pl.joins.size();
     a.n_colscan = (uint32_t)pl.colscan.size();

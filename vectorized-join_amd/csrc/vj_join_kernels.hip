@@ -290,6 +290,19 @@ __global__ void k_reset_probe(PipeArgs a) {
     for (uint32_t j = threadIdx.x; j < a.n_joins; j += blockDim.x) a.out_count[j] = 0;
 }
 
+// Task -> (join, partition) for the first taskmap_cap tasks, so the
+// build+probe kernel finds a task's partition with one load.
+__global__ __launch_bounds__(1024) void k_taskmap(PipeArgs a) {
+    const uint32_t jn = blockIdx.x, q = blockIdx.y * blockDim.x + threadIdx.x;
+    const JoinDesc &jd = a.joins[jn];
+    if (q >= jd.nparts) return;
+    const unsigned long long *pt = a.ptask + jd.ptask_off;
+    const unsigned long long t0 = a.task0[jn];
+    const unsigned long long lo = t0 + pt[q], hi = t0 + pt[q + 1];
+    const unsigned long long e = ((unsigned long long)jn << 32) | q;
+    for (unsigned long long t = lo; t < hi && t < a.taskmap_cap; t++) a.taskmap[t] = e;
+}
+
 __global__ __launch_bounds__(1024) void k_scan2(PipeArgs a) {
     __shared__ unsigned long long sh64[20];
     block_scan_array<unsigned long long>(a.join_tasks, a.task0, a.n_joins, sh64);
@@ -720,19 +733,30 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
     const unsigned long long total = a.ctl->task_total;
     const uint32_t tag_shift = hc.bits - 16;
 
+    // Lane 0 of wave 0 holds the ticket of the block's next task, fetched
+    // while the current one runs.
+    unsigned long long next_task = ~0ull;
     while (true) {
         if (w == 0) {
             unsigned long long task = 0;
-            if (lane == 0) task = atomicAdd(&a.ctl->task_ticket, 1ull);
+            if (lane == 0) task = next_task != ~0ull ? next_task : atomicAdd(&a.ctl->task_ticket, 1ull);
             task = __shfl(task, 0, 64);
             if (task >= total) {
                 if (lane == 0) t.done = 1;
             } else {
-                const uint32_t jn = wave_upper_index(a.task0, a.n_joins, task);
+                uint32_t jn, q;
+                if (task < a.taskmap_cap) {  // wave-uniform
+                    const unsigned long long e = a.taskmap[task];
+                    jn = (uint32_t)(e >> 32);
+                    q = (uint32_t)e;
+                } else {
+                    jn = wave_upper_index(a.task0, a.n_joins, task);
+                    const JoinDesc &jd0 = a.joins[jn];
+                    q = wave_upper_index(a.ptask + jd0.ptask_off, jd0.nparts, task - a.task0[jn]);
+                }
                 const JoinDesc &jd = a.joins[jn];
                 const unsigned long long *pt = a.ptask + jd.ptask_off;
                 const unsigned long long local = task - a.task0[jn];
-                const uint32_t q = wave_upper_index(pt, jd.nparts, local);
                 if (lane == 0) {
                     const JobDesc &jr = a.jobs[jd.job_r];
                     const JobDesc &js = a.jobs[jd.job_s];
@@ -773,6 +797,7 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
         }
         __syncthreads();
         if (t.done) break;
+        if (w == 0 && lane == 0) next_task = atomicAdd(&a.ctl->task_ticket, 1ull);
 
         // build: load the R chunk, then insert (index + 1, 16-bit tag) with linear probing
         const uint32_t r_lo = t.r_lo, r_n = t.r_n, nslots = t.nslots, smask = nslots - 1, shift = t.shift;
@@ -903,7 +928,12 @@ hipError_t launch_scan(const PipeArgs &a, hipStream_t s) {
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, s, a);
-    return hipGetLastError();
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (a.n_joins && a.taskmap_cap) {
+        hipLaunchKernelGGL(k_taskmap, dim3(a.n_joins, (a.max_nparts + 1023) / 1024), dim3(1024), 0, s, a);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 hipError_t launch_reset_probe(const PipeArgs &a, hipStream_t s) {

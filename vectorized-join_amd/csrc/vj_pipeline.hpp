@@ -110,6 +110,9 @@ struct PipeArgs {
     uint32_t *counts;               // count arena: tilepre / m1 / m2 of every job
     uint32_t *base;
     unsigned long long *ptask;
+    unsigned long long *taskmap;    // [taskmap_cap] task -> join << 32 | partition (tasks past it are searched)
+    unsigned long long taskmap_cap;
+    uint32_t max_nparts;            // partitions of the largest join
     uint32_t n_jobs, n_joins, n_colscan;
     uint32_t total_hist_blocks, total_p1_tiles, total_p2_chunks;
     uint32_t max_hist_bins;         // LDS bins of the histogram kernel

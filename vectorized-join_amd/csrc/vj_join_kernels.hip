@@ -507,17 +507,53 @@ struct ProbeTask {
 constexpr int kProbeUnroll = 4;
 constexpr uint16_t kNoMatch = 0xFFFF;
 
+// Wave-uniform values read from LDS, moved into scalar registers so address
+// arithmetic and loop bounds use SGPRs.
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uniform(uint64_t v) {
+    return (uint64_t)uniform((uint32_t)v) | ((uint64_t)uniform((uint32_t)(v >> 32)) << 32);
+}
+template <typename P> __device__ __forceinline__ P *uniform(P *p) {
+    return reinterpret_cast<P *>(uniform((uint64_t)reinterpret_cast<uintptr_t>(p)));
+}
+
+// One task's parameters for the walks (all wave-uniform).
+struct WalkArgs {
+    TView s;
+    uint32_t *out_r, *out_s;
+    uint64_t cap;
+    uint32_t s_lo, s_end, smask, shift, ob1;
+    bool dup;
+};
+
+__device__ __forceinline__ WalkArgs walk_args(const ProbeTask &t) {
+    WalkArgs w;
+    for (int c = 0; c < 4; c++) w.s.cols[c] = uniform(t.s.cols[c]);
+    w.s.rowids = uniform(t.s.rowids);
+    w.s.n = 0;
+    w.out_r = uniform(t.out_r);
+    w.out_s = uniform(t.out_s);
+    w.cap = uniform((uint64_t)t.cap);
+    w.s_lo = uniform(t.s_lo);
+    w.s_end = uniform(t.s_lo + t.s_n);
+    w.smask = uniform(t.nslots - 1);
+    w.shift = uniform(t.shift);
+    w.ob1 = uniform(t.one_based);
+    w.dup = uniform(t.dup) != 0;
+    return w;
+}
+
 // Writes the wave's staged pairs to rows [pos, pos + wn) of the join output.
 __device__ __forceinline__ void flush_wave(const uint32_t *obr, const uint32_t *obs, uint32_t wn,
-                                           unsigned long long pos, const ProbeTask &t) {
+                                           unsigned long long pos, const WalkArgs &wa) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const uint32_t lane = lane_id();
     for (uint32_t k = lane; k < wn; k += 64) {
         const unsigned long long o = pos + k;
-        if (o < t.cap) {
-            t.out_r[o] = obr[k];
-            t.out_s[o] = obs[k];
+        if (o < wa.cap) {
+            wa.out_r[o] = obr[k];
+            wa.out_s[o] = obs[k];
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -528,12 +564,12 @@ struct Emitter {
     uint32_t *obr, *obs;
     uint32_t wn;
     unsigned long long pos;
-    __device__ __forceinline__ void put(bool m, uint32_t r, uint32_t s, const ProbeTask &t) {
+    __device__ __forceinline__ void put(bool m, uint32_t r, uint32_t s, const WalkArgs &wa) {
         const uint64_t mb = __ballot(m);
         if (!mb) return;
         const uint32_t c = __builtin_popcountll(mb);
         if (wn + c > (uint32_t)kOutBuf) {
-            flush_wave(obr, obs, wn, pos, t);
+            flush_wave(obr, obs, wn, pos, wa);
             pos += wn;
             wn = 0;
         }
@@ -544,8 +580,8 @@ struct Emitter {
         }
         wn += c;
     }
-    __device__ __forceinline__ void finish(const ProbeTask &t) {
-        if (wn) flush_wave(obr, obs, wn, pos, t);
+    __device__ __forceinline__ void finish(const WalkArgs &wa) {
+        if (wn) flush_wave(obr, obs, wn, pos, wa);
         pos += wn;
         wn = 0;
     }
@@ -560,10 +596,11 @@ struct Emitter {
 // replay_walk then writes the kept matches without probing. All walks visit
 // rows in the same order, so the counts of the first walk are the row counts
 // of the second. Probing runs in rounds: each round reads one table slot for
-// every unfinished row of the lane, so kProbeUnroll independent LDS chains
-// are in flight per lane. The S keys of the next iteration are loaded before
-// the current one is probed (the caller loads iteration 0), so one HBM round
-// trip overlaps each iteration's probing (where registers allow, see PF).
+// every unfinished row of the lane, so U independent LDS chains are in flight
+// per lane; a round is branch-free (selects, no per-row branches). The S keys
+// of the next iteration are loaded before the current one is probed, so one
+// HBM round trip overlaps each iteration's probing (where registers allow,
+// see PF).
 enum { WALK_COUNT = 0, WALK_EMIT = 1, WALK_RECORD = 2 };
 
 template <int N> __device__ __forceinline__ uint32_t res_get(const uint32_t (&r)[N], int k) {
@@ -576,93 +613,91 @@ template <int N> __device__ __forceinline__ void res_set(uint32_t (&r)[N], int k
 constexpr int kProbeIters = kProbeChunk / (kProbeThreads * kProbeUnroll);
 static_assert(kProbeIters * kProbeThreads * kProbeUnroll == kProbeChunk, "probe chunk shape");
 
-// First S row of this lane in iteration `it` of a walk.
-__device__ __forceinline__ uint32_t walk_row(const ProbeTask &t, int it) {
-    return t.s_lo + it * kProbeThreads * kProbeUnroll + (threadIdx.x >> 6) * 64 * kProbeUnroll + lane_id();
+// Rows per lane per iteration of the duplicate-chunk walks (COUNT, EMIT):
+// fewer than kProbeUnroll keep their registers within the kernel's budget.
+constexpr int kDupUnroll = 2;
+
+// First S row of this lane in iteration `it` of a walk with U rows per lane.
+template <int U> __device__ __forceinline__ uint32_t walk_row(const WalkArgs &wa, int it) {
+    return wa.s_lo + it * kProbeThreads * U + (threadIdx.x >> 6) * 64 * U + lane_id();
+}
+// First S row of the wave in iteration `it` (wave-uniform).
+template <int U> __device__ __forceinline__ uint32_t walk_wave_row(const WalkArgs &wa, int it) {
+    return wa.s_lo + it * kProbeThreads * U + uniform((uint32_t)(threadIdx.x >> 6)) * 64 * U;
 }
 
 // Loads this lane's S keys of iteration `it` (rows past the chunk are left alone).
-template <int KW, int NC>
-__device__ __forceinline__ void walk_load(const ProbeTask &t, int it, Key<KW, NC> (&k)[kProbeUnroll]) {
-    const uint32_t row = walk_row(t, it), s_end = t.s_lo + t.s_n;
+template <int U, int KW, int NC>
+__device__ __forceinline__ void walk_load(const WalkArgs &wa, int it, Key<KW, NC> (&k)[U]) {
+    const uint32_t row = walk_row<U>(wa, it);
 #pragma unroll
-    for (int u = 0; u < kProbeUnroll; u++)
-        if (row + u * 64 < s_end) k[u] = load_key<KW, NC>(t.s, row + u * 64);
+    for (int u = 0; u < U; u++)
+        if (row + u * 64 < wa.s_end) k[u] = load_key<KW, NC>(wa.s, row + u * 64);
 }
 
 template <int MODE, int HP, int KW, int NC>
-__device__ __forceinline__ uint32_t probe_walk(const ProbeTask &t, const HashCfg &hc, uint32_t tag_shift,
+__device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg &hc, uint32_t tag_shift,
                                                const typename KeyT<KW>::T *rk, const uint32_t *rid,
                                                const uint32_t *slots, uint32_t (&res)[kProbeIters * kProbeUnroll / 2],
-                                               Key<KW, NC> (&nk)[kProbeUnroll], Emitter &em) {
+                                               Emitter &em) {
     constexpr int CAP = build_cap(KW * NC);
-    constexpr int U = kProbeUnroll;
+    constexpr int U = MODE == WALK_RECORD ? kProbeUnroll : kDupUnroll;
+    constexpr int ITERS = kProbeChunk / (kProbeThreads * U);
+    static_assert(ITERS * kProbeThreads * U == kProbeChunk, "walk shape");
     // Next-iteration loads double the key registers; they are used where the
     // kernel stays within its register budget (single 8-byte key, FNV / XOR).
     constexpr bool PF = MODE != WALK_EMIT && KW * NC <= 8 && HP != HP_GENERIC;
-    const uint32_t smask = t.nslots - 1, shift = t.shift, ob1 = t.one_based;
-    const bool dup = t.dup != 0;
-    const uint32_t s_end = t.s_lo + t.s_n;
+    const uint32_t smask = wa.smask, shift = wa.shift, s_end = wa.s_end;
     uint32_t total = 0;
+    Key<KW, NC> nk[U];
+    walk_load<U, KW, NC>(wa, 0, nk);
 #pragma unroll
-    for (int it = 0; it < kProbeIters; it++) {
-        const uint32_t row = walk_row(t, it);
-        if (row - lane_id() >= s_end) break;  // wave-uniform
+    for (int it = 0; it < ITERS; it++) {
+        if (walk_wave_row<U>(wa, it) >= s_end) break;  // wave-uniform
+        const uint32_t row = walk_row<U>(wa, it);
         Key<KW, NC> sk[U];
-        uint32_t sid[U], sl[U], tag[U];
-        uint32_t act = 0;  // bit u: row u of this lane still probing
-        if (!PF && it > 0) walk_load<KW, NC>(t, it, nk);
+        uint32_t sid[U], st[U];  // st: 16-bit tag << 16 | slot (slots < 2^16)
+        uint32_t act = 0;        // bit u: row u of this lane still probing
+        if (!PF && it > 0) walk_load<U, KW, NC>(wa, it, nk);
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const uint32_t i = row + u * 64;
             sk[u] = nk[u];
             sid[u] = 0;
             if (MODE == WALK_RECORD) res_set(res, it * U + u, kNoMatch);
-            if (i < s_end) {
-                if (MODE == WALK_EMIT) sid[u] = t.s.rowids ? t.s.rowids[i] : i;
-                act |= 1u << u;
-            }
+            if (MODE == WALK_EMIT && i < s_end) sid[u] = wa.s.rowids ? wa.s.rowids[i] : i;
+            act |= (i < s_end ? 1u : 0u) << u;
         }
-        if (PF && it + 1 < kProbeIters) walk_load<KW, NC>(t, it + 1, nk);
+        if (PF && it + 1 < ITERS) walk_load<U, KW, NC>(wa, it + 1, nk);
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            sl[u] = 0;
-            tag[u] = 0;
-            if (act & (1u << u)) {
-                const uint64_t h = hash_key<HP, KW, NC>(sk[u], hc);
-                tag[u] = (uint32_t)(h >> tag_shift) & 0xFFFFu;
-                sl[u] = (uint32_t)(h >> shift) & smask;
-            }
+            const uint64_t h = hash_key<HP, KW, NC>(sk[u], hc);
+            st[u] = (((uint32_t)(h >> tag_shift) & 0xFFFFu) << 16) | ((uint32_t)(h >> shift) & smask);
         }
         while (__ballot(act != 0)) {
             uint32_t e[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) e[u] = (act & (1u << u)) ? slots[sl[u]] : 0u;
+            for (int u = 0; u < U; u++) e[u] = slots[st[u] & 0xFFFFu];  // always a valid slot
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                bool m = false;
-                uint32_t o = 0;
-                if (act & (1u << u)) {
-                    if (e[u] == 0) {
-                        act &= ~(1u << u);
-                    } else {
-                        if ((e[u] >> 16) == tag[u]) {
-                            o = (e[u] & 0xFFFFu) - 1;
-                            bool eq = true;
+                const bool live = (act >> u) & 1u;
+                const bool hit_tag = e[u] != 0 && (e[u] >> 16) == (st[u] >> 16);
+                const uint32_t o = hit_tag ? (e[u] & 0xFFFFu) - 1 : 0u;
+                bool eq = hit_tag;
 #pragma unroll
-                            for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == sk[u].c[c];
-                            if (eq) {
-                                m = true;
-                                if (!dup) act &= ~(1u << u);
-                            }
-                        }
-                        sl[u] = (sl[u] + 1) & smask;
-                    }
-                }
+                for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == sk[u].c[c];
+                const bool m = live && eq;
+                // a row is done at an empty slot, or at its (only) match
+                const bool stop = e[u] == 0 || (eq && !wa.dup);
+                act &= ~((live && stop ? 1u : 0u) << u);
+                st[u] = (st[u] & 0xFFFF0000u) | ((st[u] + 1) & smask);
                 if (MODE == WALK_EMIT) {
-                    em.put(m, rid[o] + ob1, sid[u] + ob1, t);
+                    em.put(m, rid[o] + wa.ob1, sid[u] + wa.ob1, wa);
                 } else {
-                    if (MODE == WALK_RECORD && m) res_set(res, it * U + u, o);
+                    if (MODE == WALK_RECORD) {
+                        const uint32_t cur = res_get(res, it * U + u);
+                        res_set(res, it * U + u, m ? o : cur);
+                    }
                     total += __builtin_popcountll(__ballot(m));
                 }
             }
@@ -674,34 +709,34 @@ __device__ __forceinline__ uint32_t probe_walk(const ProbeTask &t, const HashCfg
 // Writes the matches a WALK_RECORD walk kept. The S row ids of a group of G
 // iterations are loaded before any is written, so their round trips overlap.
 template <int G>
-__device__ __forceinline__ void replay_walk(const ProbeTask &t, const uint32_t *rid,
+__device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint32_t *rid,
                                             const uint32_t (&res)[kProbeIters * kProbeUnroll / 2], Emitter &em) {
     constexpr int U = kProbeUnroll;
     static_assert(kProbeIters % G == 0, "replay groups");
-    const uint32_t ob1 = t.one_based, s_end = t.s_lo + t.s_n;
+    const uint32_t ob1 = wa.ob1, s_end = wa.s_end;
 #pragma unroll
     for (int g = 0; g < kProbeIters; g += G) {
-        if (walk_row(t, g) - lane_id() >= s_end) break;  // wave-uniform
+        if (walk_wave_row<U>(wa, g) >= s_end) break;  // wave-uniform
         uint32_t sid[G * U];
 #pragma unroll
         for (int it = 0; it < G; it++) {
-            const uint32_t row = walk_row(t, g + it);
+            const uint32_t row = walk_row<U>(wa, g + it);
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const uint32_t i = row + u * 64;
                 sid[it * U + u] = i;
-                if (t.s.rowids && i < s_end && res_get(res, (g + it) * U + u) != kNoMatch)
-                    sid[it * U + u] = t.s.rowids[i];
+                if (wa.s.rowids && i < s_end && res_get(res, (g + it) * U + u) != kNoMatch)
+                    sid[it * U + u] = wa.s.rowids[i];
             }
         }
 #pragma unroll
         for (int it = 0; it < G; it++) {
-            if (walk_row(t, g + it) - lane_id() >= s_end) break;  // wave-uniform
+            if (walk_wave_row<U>(wa, g + it) >= s_end) break;  // wave-uniform
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const uint32_t o = res_get(res, (g + it) * U + u);
                 const bool m = o != kNoMatch;
-                em.put(m, (m ? rid[o] : 0u) + ob1, sid[it * U + u] + ob1, t);
+                em.put(m, rid[m ? o : 0u] + ob1, sid[it * U + u] + ob1, wa);
             }
         }
     }
@@ -782,7 +817,9 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
                     t.s_lo = sb[q] + sc * jd.cap_s;
                     t.s_n = ns - sc * jd.cap_s < jd.cap_s ? ns - sc * jd.cap_s : jd.cap_s;
                     uint32_t ns2 = 64;
-                    while (ns2 < 2 * t.r_n) ns2 <<= 1;
+                    // load <= 1/4 where the slot array (2 * CAP) allows, else <= 1/2:
+                    // a probe round costs the wave's longest chain, not the mean
+                    while (ns2 < 4 * t.r_n && ns2 < 2 * (uint32_t)CAP) ns2 <<= 1;
                     t.nslots = ns2;
                     t.shift = jd.slot_shift;
                     t.one_based = jd.one_based;
@@ -834,11 +871,10 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
 
         // count (recording matches when the chunk has no duplicate keys), reserve
         // the task's output rows with one atomic, then write them
-        const bool dup = t.dup != 0;
-        Key<KW, NC> nk[kProbeUnroll];
-        walk_load<KW, NC>(t, 0, nk);
-        const uint32_t wc = dup ? probe_walk<WALK_COUNT, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, res, nk, em)
-                                : probe_walk<WALK_RECORD, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, res, nk, em);
+        const WalkArgs wa = walk_args(t);
+        const bool dup = wa.dup;
+        const uint32_t wc = dup ? probe_walk<WALK_COUNT, HP, KW, NC>(wa, hc, tag_shift, rk, rid, slots, res, em)
+                                : probe_walk<WALK_RECORD, HP, KW, NC>(wa, hc, tag_shift, rk, rid, slots, res, em);
         if (lane == 0) t.wave_count[w] = wc;
         __syncthreads();
         if (tid == 0) {
@@ -852,13 +888,12 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
             em.wn = 0;
             for (uint32_t k = 0; k < w; k++) em.pos += t.wave_count[k];
             if (dup) {
-                walk_load<KW, NC>(t, 0, nk);
-                probe_walk<WALK_EMIT, HP, KW, NC>(t, hc, tag_shift, rk, rid, slots, res, nk, em);
+                probe_walk<WALK_EMIT, HP, KW, NC>(wa, hc, tag_shift, rk, rid, slots, res, em);
             } else {
                 // row-id groups of four iterations where registers allow (single 8-byte key, FNV / XOR)
-                replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 4 : 1>(t, rid, res, em);
+                replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 4 : 1>(wa, rid, res, em);
             }
-            em.finish(t);
+            em.finish(wa);
         }
         __syncthreads();
     }

@@ -14,8 +14,9 @@
 //               precomputed destinations; no block waits on another
 //   k_probe     persistent; (join, partition, R-chunk, S-chunk) tasks from a
 //               device queue: open-addressing table of the R chunk in LDS,
-//               probe with the S chunk, 64-lane ballot compaction of matches
-//               into per-wave LDS buffers, one global atomic per buffer flush
+//               probe with the S chunk; matches counted first, one output
+//               reservation per task, then written with 64-lane ballot
+//               compaction straight into contiguous output rows
 //
 // The reference's counterparts are histogram_kernel / prefix_sum_kernel /
 // element_swap_kernel (src/join/join_provider.cu:315-380) and the build /
@@ -543,47 +544,21 @@ __device__ __forceinline__ WalkArgs walk_args(const ProbeTask &t) {
     return w;
 }
 
-// Writes the wave's staged pairs to rows [pos, pos + wn) of the join output.
-__device__ __forceinline__ void flush_wave(const uint32_t *obr, const uint32_t *obs, uint32_t wn,
-                                           unsigned long long pos, const WalkArgs &wa) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    const uint32_t lane = lane_id();
-    for (uint32_t k = lane; k < wn; k += 64) {
-        const unsigned long long o = pos + k;
-        if (o < wa.cap) {
-            wa.out_r[o] = obr[k];
-            wa.out_s[o] = obs[k];
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-}
-
-// Per-wave staging of output pairs; rows go out in buffer-sized runs.
+// Writes a wave's matches straight to the join output: the matching lanes of
+// one ballot take consecutive rows from `pos`, so every store instruction
+// covers one contiguous run of each output column.
 struct Emitter {
-    uint32_t *obr, *obs;
-    uint32_t wn;
     unsigned long long pos;
     __device__ __forceinline__ void put(bool m, uint32_t r, uint32_t s, const WalkArgs &wa) {
         const uint64_t mb = __ballot(m);
-        if (!mb) return;
-        const uint32_t c = __builtin_popcountll(mb);
-        if (wn + c > (uint32_t)kOutBuf) {
-            flush_wave(obr, obs, wn, pos, wa);
-            pos += wn;
-            wn = 0;
-        }
         if (m) {
-            const uint32_t p = wn + lanes_below(mb);
-            obr[p] = r;
-            obs[p] = s;
+            const unsigned long long o = pos + lanes_below(mb);
+            if (o < wa.cap) {
+                wa.out_r[o] = r;
+                wa.out_s[o] = s;
+            }
         }
-        wn += c;
-    }
-    __device__ __forceinline__ void finish(const WalkArgs &wa) {
-        if (wn) flush_wave(obr, obs, wn, pos, wa);
-        pos += wn;
-        wn = 0;
+        pos += __builtin_popcountll(mb);
     }
 };
 
@@ -756,14 +731,12 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
     KT *rk = reinterpret_cast<KT *>(smem);                       // [NC][CAP]
     uint32_t *rid = reinterpret_cast<uint32_t *>(rk + NC * CAP);  // [CAP]
     uint32_t *slots = rid + CAP;                                  // [2 * CAP]
-    uint32_t *ob = slots + 2 * CAP;                               // [kProbeWaves][2][kOutBuf]
     __shared__ ProbeTask t;
     uint32_t res[kProbeIters * kProbeUnroll / 2];  // matched build row per probe row, two per word
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     Emitter em;
-    em.obr = ob + w * 2 * kOutBuf;
-    em.obs = em.obr + kOutBuf;
+    em.pos = 0;
     const HashCfg hc = a.hash;
     const unsigned long long total = a.ctl->task_total;
     const uint32_t tag_shift = hc.bits - 16;
@@ -885,7 +858,6 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
         __syncthreads();
         if (wc) {
             em.pos = t.out_base;
-            em.wn = 0;
             for (uint32_t k = 0; k < w; k++) em.pos += t.wave_count[k];
             if (dup) {
                 probe_walk<WALK_EMIT, HP, KW, NC>(wa, hc, tag_shift, rk, rid, slots, res, em);
@@ -893,7 +865,6 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
                 // row-id groups of four iterations where registers allow (single 8-byte key, FNV / XOR)
                 replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 4 : 1>(wa, rid, res, em);
             }
-            em.finish(wa);
         }
         __syncthreads();
     }
@@ -931,7 +902,7 @@ size_t pass_lds_bytes(int kw, int nc) {
 
 size_t probe_lds_bytes(int kw, int nc) {
     const int kb = kw * nc, cap = build_cap(kb);
-    return (size_t)cap * kb + (size_t)cap * 4 + (size_t)cap * 8 + (size_t)kProbeWaves * 2 * kOutBuf * 4;
+    return (size_t)cap * kb + (size_t)cap * 4 + (size_t)cap * 8;
 }
 
 }  // namespace

@@ -38,7 +38,6 @@ constexpr int kProbeThreads = 512;  // build+probe kernel
 constexpr int kProbeWaves = kProbeThreads / 64;
 constexpr int kMaxPassBits = 8;     // digits per pass <= 256
 constexpr int kMaxHistBits = 14;    // 2D histogram in LDS: 2^14 x 4 B = 64 KiB
-constexpr int kOutBuf = 128;        // per-wave LDS output staging (pairs)
 constexpr uint32_t kXcds = 8;       // accelerator dies (XCDs) of an MI355X
 
 // Tuples per pass tile, per build chunk and per probe chunk, by key bytes per row.

@@ -32,6 +32,18 @@ namespace {
 
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
+// Wave-uniform values read from LDS, moved into scalar registers so address
+// arithmetic and loop bounds use SGPRs. readfirstlane returns a signed int:
+// it is taken as uint32_t before any widening, so a 64-bit value's halves are
+// zero-extended (sign extension would corrupt the high word).
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ uint64_t uniform(uint64_t v) {
+    return (uint64_t)uniform((uint32_t)v) | ((uint64_t)uniform((uint32_t)(v >> 32)) << 32);
+}
+template <typename P> __device__ __forceinline__ P *uniform(P *p) {
+    return reinterpret_cast<P *>(uniform((uint64_t)reinterpret_cast<uintptr_t>(p)));
+}
+
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
@@ -406,6 +418,10 @@ __global__ __launch_bounds__(kPassThreads, 6) void k_pass(PipeArgs a, int pass) 
 
     const uint32_t bins = ti.bins, shift = ti.shift, mask = bins - 1;
     const TView src = ti.src;
+    KT *dcols[NC];  // destination columns, in scalar registers
+#pragma unroll
+    for (int c = 0; c < NC; c++) dcols[c] = static_cast<KT *>(uniform(ti.dst_cols[c]));
+    uint32_t *const dids = uniform(ti.dst_ids);
     const HashCfg hc = a.hash;
     for (uint32_t lo = ti.lo; lo < ti.hi; lo += T) {
         const uint32_t hi = lo + T < ti.hi ? lo + T : ti.hi;
@@ -466,22 +482,23 @@ __global__ __launch_bounds__(kPassThreads, 6) void k_pass(PipeArgs a, int pass) 
             }
         }
         __syncthreads();
+        // 4. write the keys in slot order
         const uint32_t n = hi - lo;
         for (uint32_t k = tid; k < n; k += kPassThreads) {
             const uint32_t d = sdig[k];
             const uint32_t dst = gdst[d] + (k - loc[d]);
 #pragma unroll
-            for (int c = 0; c < NC; c++) static_cast<KT *>(ti.dst_cols[c])[dst] = skeys[c * T + k];
+            for (int c = 0; c < NC; c++) dcols[c][dst] = skeys[c * T + k];
         }
         __syncthreads();
-        // 4. the row ids, through the same staging buffer
+        // 5. the row ids, through the same staging buffer
 #pragma unroll
         for (int it = 0; it < ITEMS; it++)
             if (slot[it] != ~0u) sids[slot[it]] = ids[it];
         __syncthreads();
         for (uint32_t k = tid; k < n; k += kPassThreads) {
             const uint32_t d = sdig[k];
-            ti.dst_ids[gdst[d] + (k - loc[d])] = sids[k];
+            dids[gdst[d] + (k - loc[d])] = sids[k];
         }
         __syncthreads();
         if (tid < bins) gdst[tid] += cnt;  // next step of a pass-2 chunk
@@ -507,16 +524,6 @@ struct ProbeTask {
 
 constexpr int kProbeUnroll = 4;
 constexpr uint16_t kNoMatch = 0xFFFF;
-
-// Wave-uniform values read from LDS, moved into scalar registers so address
-// arithmetic and loop bounds use SGPRs.
-__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ uint64_t uniform(uint64_t v) {
-    return (uint64_t)uniform((uint32_t)v) | ((uint64_t)uniform((uint32_t)(v >> 32)) << 32);
-}
-template <typename P> __device__ __forceinline__ P *uniform(P *p) {
-    return reinterpret_cast<P *>(uniform((uint64_t)reinterpret_cast<uintptr_t>(p)));
-}
 
 // One task's parameters for the walks (all wave-uniform).
 struct WalkArgs {

@@ -245,12 +245,12 @@ void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0
         pl.counts_words += (uint64_t)jd.p1_tiles * n1;
         jd.m1_off = pl.counts_words;
         pl.counts_words += (uint64_t)(jd.nseg + 1) * n1;
-        for (uint32_t c0 = 0; c0 < n1; c0 += 256) pl.colscan.push_back({job_index, 1u, c0, 0u});
+        for (uint32_t c0 = 0; c0 < n1; c0 += kColScanCols) pl.colscan.push_back({job_index, 1u, c0, 0u});
         if (jd.b2) {
             pl.any_p2 = true;
             jd.m2_off = pl.counts_words;
             pl.counts_words += (uint64_t)(jd.nseg + 1) * nb;
-            for (uint32_t c0 = 0; c0 < nb; c0 += 256) pl.colscan.push_back({job_index, 2u, c0, 0u});
+            for (uint32_t c0 = 0; c0 < nb; c0 += kColScanCols) pl.colscan.push_back({job_index, 2u, c0, 0u});
             chunks = n1 * jd.nseg;
         }
         pl.max_hist_bins = std::max(pl.max_hist_bins, (jd.b2 ? nb : 0) + n1 * (1 + kHistThreads / 64));

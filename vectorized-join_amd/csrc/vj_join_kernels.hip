@@ -109,19 +109,33 @@ template <typename T> __device__ T block_excl_scan(T v, T *sh, T *total) {
     return r;
 }
 
-// Exclusive scan of in[0..n) into out[0..n], total into out[n]; in may equal out.
+// Exclusive scan of in[0..n) into out[0..n], total into out[n]; in may equal
+// out. Each thread takes a contiguous run of elements, read in register
+// chunks of kScanChunk so the loads of a chunk are in flight together.
+constexpr int kScanChunk = 16;
 template <typename T> __device__ void block_scan_array(const T *in, T *out, uint32_t n, T *sh) {
     const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
     const uint32_t lo = threadIdx.x * per < n ? threadIdx.x * per : n;
     const uint32_t hi = lo + per < n ? lo + per : n;
     T sum = 0;
-    for (uint32_t i = lo; i < hi; i++) sum += in[i];
+    for (uint32_t b = lo; b < hi; b += kScanChunk) {
+        T v[kScanChunk];
+#pragma unroll
+        for (int r = 0; r < kScanChunk; r++) v[r] = b + r < hi ? in[b + r] : T(0);
+#pragma unroll
+        for (int r = 0; r < kScanChunk; r++) sum += v[r];
+    }
     T total;
     T pre = block_excl_scan<T>(sum, sh, &total);
-    for (uint32_t i = lo; i < hi; i++) {
-        T v = in[i];
-        out[i] = pre;
-        pre += v;
+    for (uint32_t b = lo; b < hi; b += kScanChunk) {
+        T v[kScanChunk];  // the whole chunk is read before any of it is written (in == out)
+#pragma unroll
+        for (int r = 0; r < kScanChunk; r++) v[r] = b + r < hi ? in[b + r] : T(0);
+#pragma unroll
+        for (int r = 0; r < kScanChunk; r++) {
+            if (b + r < hi) out[b + r] = pre;
+            pre += v[r];
+        }
     }
     if (threadIdx.x == 0) out[n] = total;
     __syncthreads();
@@ -214,35 +228,42 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(PipeArgs a) {
     }
 }
 
-// Exclusive scans down 256 columns of a [nseg + 1][ncols] count matrix (the
-// last row receives the column totals). 1024 threads = 256 columns x 4 row
-// groups; each thread sums its rows, the groups' partial sums are combined
-// in LDS, and the rows are rewritten with their prefixes.
+// Exclusive scans down kColScanCols columns of a [nseg + 1][ncols] count
+// matrix (the last row receives the column totals). 1024 threads = 64 columns
+// x 16 row groups of at most 16 rows (nseg <= 256): each thread reads its
+// rows into registers, the groups' sums are combined in LDS, and the rows are
+// rewritten with their prefixes.
+constexpr int kColScanGroups = 1024 / kColScanCols;
+static_assert(kColScanGroups * 16 >= 256, "column scan covers 256 segments");
 __global__ __launch_bounds__(1024) void k_colscan(PipeArgs a) {
-    __shared__ uint32_t part[4][256];
+    __shared__ uint32_t part[kColScanGroups][kColScanCols];
     const ColScanDesc d = a.colscan[blockIdx.x];
     const JobDesc &job = a.jobs[d.job];
     const uint32_t ncols = d.which == 1 ? (1u << job.b1) : (1u << (job.b1 + job.b2));
     uint32_t *m = a.counts + (d.which == 1 ? job.m1_off : job.m2_off);
-    const uint32_t c = d.col0 + (threadIdx.x & 255), rg = threadIdx.x >> 8;
-    const uint32_t rows = job.nseg, per = (rows + 3) / 4;
+    const uint32_t cc = threadIdx.x % kColScanCols, rg = threadIdx.x / kColScanCols;
+    const uint32_t c = d.col0 + cc;
+    const uint32_t rows = job.nseg, per = (rows + kColScanGroups - 1) / kColScanGroups;  // <= 16
     const uint32_t lo = rg * per < rows ? rg * per : rows, hi = lo + per < rows ? lo + per : rows;
     const bool col_ok = c < ncols;
+    uint32_t v[16];
     uint32_t sum = 0;
-    if (col_ok)
-        for (uint32_t r = lo; r < hi; r++) sum += m[(uint64_t)r * ncols + c];
-    part[rg][threadIdx.x & 255] = sum;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        v[r] = col_ok && lo + r < hi ? m[(uint64_t)(lo + r) * ncols + c] : 0u;
+        sum += v[r];
+    }
+    part[rg][cc] = sum;
     __syncthreads();
     uint32_t pre = 0;
-    for (uint32_t k = 0; k < rg; k++) pre += part[k][threadIdx.x & 255];
+    for (uint32_t k = 0; k < rg; k++) pre += part[k][cc];
     if (col_ok) {
-        for (uint32_t r = lo; r < hi; r++) {
-            const uint64_t at = (uint64_t)r * ncols + c;
-            const uint32_t v = m[at];
-            m[at] = pre;
-            pre += v;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            if (lo + r < hi) m[(uint64_t)(lo + r) * ncols + c] = pre;
+            pre += v[r];
         }
-        if (rg == 3) m[(uint64_t)rows * ncols + c] = pre;
+        if (rg == kColScanGroups - 1) m[(uint64_t)rows * ncols + c] = pre;
     }
 }
 
@@ -277,7 +298,12 @@ __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
 }
 
 // One block of 1024 threads per join: build/probe tasks = R chunks x S chunks
-// of every partition pair, as an exclusive prefix over partitions.
+// of every partition pair, as an exclusive prefix over partitions. Each thread
+// takes a contiguous run of partitions; their bases are read in register chunks.
+__device__ __forceinline__ unsigned long long part_tasks(uint32_t nr, uint32_t ns, const JoinDesc &jd) {
+    return nr && ns ? (unsigned long long)((nr + jd.cap_r - 1) / jd.cap_r) * ((ns + jd.cap_s - 1) / jd.cap_s) : 0ull;
+}
+
 __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
     __shared__ unsigned long long sh64[20];
     const uint32_t jn = blockIdx.x;
@@ -285,15 +311,43 @@ __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
     const uint32_t *rb = a.base + a.jobs[jd.job_r].base_off;
     const uint32_t *sb = a.base + a.jobs[jd.job_s].base_off;
     unsigned long long *pt = a.ptask + jd.ptask_off;
-    for (uint32_t q = threadIdx.x; q < jd.nparts; q += blockDim.x) {
-        const uint32_t nr = rb[q + 1] - rb[q], ns = sb[q + 1] - sb[q];
-        unsigned long long t = 0;
-        if (nr && ns) t = (unsigned long long)((nr + jd.cap_r - 1) / jd.cap_r) * ((ns + jd.cap_s - 1) / jd.cap_s);
-        pt[q] = t;
+    const uint32_t n = jd.nparts, per = (n + blockDim.x - 1) / blockDim.x;
+    const uint32_t lo = threadIdx.x * per < n ? threadIdx.x * per : n;
+    const uint32_t hi = lo + per < n ? lo + per : n;
+    constexpr int C = kScanChunk;
+    unsigned long long sum = 0;
+    for (uint32_t b = lo; b < hi; b += C) {
+        uint32_t r[C + 1], q[C + 1];
+#pragma unroll
+        for (int k = 0; k <= C; k++) {
+            r[k] = b + k <= hi ? rb[b + k] : 0u;
+            q[k] = b + k <= hi ? sb[b + k] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < C; k++)
+            if (b + k < hi) sum += part_tasks(r[k + 1] - r[k], q[k + 1] - q[k], jd);
     }
-    __syncthreads();
-    block_scan_array<unsigned long long>(pt, pt, jd.nparts, sh64);
-    if (threadIdx.x == 0) a.join_tasks[jn] = pt[jd.nparts];
+    unsigned long long total;
+    unsigned long long pre = block_excl_scan<unsigned long long>(sum, sh64, &total);
+    for (uint32_t b = lo; b < hi; b += C) {
+        uint32_t r[C + 1], q[C + 1];
+#pragma unroll
+        for (int k = 0; k <= C; k++) {
+            r[k] = b + k <= hi ? rb[b + k] : 0u;
+            q[k] = b + k <= hi ? sb[b + k] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+            if (b + k < hi) {
+                pt[b + k] = pre;
+                pre += part_tasks(r[k + 1] - r[k], q[k + 1] - q[k], jd);
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        pt[n] = total;
+        a.join_tasks[jn] = total;
+    }
 }
 
 // Resets the queue and the output counters to run build+probe again (after

@@ -81,7 +81,9 @@ struct JoinDesc {
     uint32_t one_based;
 };
 
-// One block of the column-scan kernel: 256 columns of one job's m1 or m2.
+constexpr int kColScanCols = 64;     // columns per block of the column-scan kernel
+
+// One block of the column-scan kernel: kColScanCols columns of one job's m1 or m2.
 struct ColScanDesc {
     uint32_t job;
     uint32_t which;  // 1: m1, 2: m2

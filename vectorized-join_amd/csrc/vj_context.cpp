@@ -167,7 +167,7 @@ Err check_pair(const vj_table *r, const vj_table *s) {
     return Err::ok();
 }
 
-constexpr uint32_t kMaxSegments = 256;  // segments (histogram blocks) per job
+constexpr uint32_t kMaxSegments = 512;  // segments (histogram blocks) per job
 
 // Bytes of a resident table of n rows.
 size_t built_bytes(uint64_t n, int nc, int kw, uint32_t b1, uint32_t b2) {
@@ -235,8 +235,11 @@ void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0
         pl.any_part = true;
         jd.p1_tiles = (uint32_t)(((uint64_t)jd.n + T - 1) / T);
         // segments: pass-2 chunks (a segment's rows of one pass-1 partition)
-        // of about one tile; for one pass, enough segments to spread the count
-        uint64_t want = jd.b2 ? ((uint64_t)jd.n + (uint64_t)n1 * T - 1) / ((uint64_t)n1 * T)
+        // averaging 15/16 of a tile, so that chunk sizes (spread around the
+        // mean) rarely need a second pass-2 step; for one pass, enough
+        // segments to spread the count
+        const uint64_t chunk = (uint64_t)T * 15 / 16;
+        uint64_t want = jd.b2 ? ((uint64_t)jd.n + (uint64_t)n1 * chunk - 1) / ((uint64_t)n1 * chunk)
                               : ((uint64_t)jd.p1_tiles + 15) / 16;
         want = std::max<uint64_t>(1, std::min<uint64_t>({want, kMaxSegments, jd.p1_tiles}));
         jd.seg_tiles = (uint32_t)((jd.p1_tiles + want - 1) / want);

@@ -230,11 +230,10 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(PipeArgs a) {
 
 // Exclusive scans down kColScanCols columns of a [nseg + 1][ncols] count
 // matrix (the last row receives the column totals). 1024 threads = 64 columns
-// x 16 row groups of at most 16 rows (nseg <= 256): each thread reads its
-// rows into registers, the groups' sums are combined in LDS, and the rows are
-// rewritten with their prefixes.
+// x 16 row groups: each thread reads its rows in register chunks of 16, the
+// groups' sums are combined in LDS, and the rows are rewritten with their
+// prefixes.
 constexpr int kColScanGroups = 1024 / kColScanCols;
-static_assert(kColScanGroups * 16 >= 256, "column scan covers 256 segments");
 __global__ __launch_bounds__(1024) void k_colscan(PipeArgs a) {
     __shared__ uint32_t part[kColScanGroups][kColScanCols];
     const ColScanDesc d = a.colscan[blockIdx.x];
@@ -243,25 +242,31 @@ __global__ __launch_bounds__(1024) void k_colscan(PipeArgs a) {
     uint32_t *m = a.counts + (d.which == 1 ? job.m1_off : job.m2_off);
     const uint32_t cc = threadIdx.x % kColScanCols, rg = threadIdx.x / kColScanCols;
     const uint32_t c = d.col0 + cc;
-    const uint32_t rows = job.nseg, per = (rows + kColScanGroups - 1) / kColScanGroups;  // <= 16
+    const uint32_t rows = job.nseg, per = (rows + kColScanGroups - 1) / kColScanGroups;
     const uint32_t lo = rg * per < rows ? rg * per : rows, hi = lo + per < rows ? lo + per : rows;
     const bool col_ok = c < ncols;
-    uint32_t v[16];
     uint32_t sum = 0;
+    for (uint32_t b = lo; b < hi; b += 16) {
+        uint32_t v[16];
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        v[r] = col_ok && lo + r < hi ? m[(uint64_t)(lo + r) * ncols + c] : 0u;
-        sum += v[r];
+        for (int r = 0; r < 16; r++) v[r] = col_ok && b + r < hi ? m[(uint64_t)(b + r) * ncols + c] : 0u;
+#pragma unroll
+        for (int r = 0; r < 16; r++) sum += v[r];
     }
     part[rg][cc] = sum;
     __syncthreads();
     uint32_t pre = 0;
     for (uint32_t k = 0; k < rg; k++) pre += part[k][cc];
     if (col_ok) {
+        for (uint32_t b = lo; b < hi; b += 16) {
+            uint32_t v[16];
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            if (lo + r < hi) m[(uint64_t)(lo + r) * ncols + c] = pre;
-            pre += v[r];
+            for (int r = 0; r < 16; r++) v[r] = b + r < hi ? m[(uint64_t)(b + r) * ncols + c] : 0u;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                if (b + r < hi) m[(uint64_t)(b + r) * ncols + c] = pre;
+                pre += v[r];
+            }
         }
         if (rg == kColScanGroups - 1) m[(uint64_t)rows * ncols + c] = pre;
     }

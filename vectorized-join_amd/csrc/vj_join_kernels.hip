@@ -582,6 +582,7 @@ struct ProbeTask {
 };
 
 constexpr int kProbeUnroll = 4;
+constexpr int kBucketSlots = 4;  // LDS table: buckets of 4 slots (one 16-byte load)
 constexpr uint16_t kNoMatch = 0xFFFF;
 
 // One task's parameters for the walks (all wave-uniform).
@@ -589,7 +590,7 @@ struct WalkArgs {
     TView s;
     uint32_t *out_r, *out_s;
     uint64_t cap;
-    uint32_t s_lo, s_end, smask, shift, ob1;
+    uint32_t s_lo, s_end, bmask, shift, ob1;
     bool dup;
 };
 
@@ -603,7 +604,7 @@ __device__ __forceinline__ WalkArgs walk_args(const ProbeTask &t) {
     w.cap = uniform((uint64_t)t.cap);
     w.s_lo = uniform(t.s_lo);
     w.s_end = uniform(t.s_lo + t.s_n);
-    w.smask = uniform(t.nslots - 1);
+    w.bmask = uniform((t.nslots >> 2) - 1);
     w.shift = uniform(t.shift);
     w.ob1 = uniform(t.one_based);
     w.dup = uniform(t.dup) != 0;
@@ -688,7 +689,7 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
     // Next-iteration loads double the key registers; they are used where the
     // kernel stays within its register budget (single 8-byte key, FNV / XOR).
     constexpr bool PF = MODE != WALK_EMIT && KW * NC <= 8 && HP != HP_GENERIC;
-    const uint32_t smask = wa.smask, shift = wa.shift, s_end = wa.s_end;
+    const uint32_t bmask = wa.bmask, shift = wa.shift, s_end = wa.s_end;
     uint32_t total = 0;
     Key<KW, NC> nk[U];
     walk_load<U, KW, NC>(wa, 0, nk);
@@ -697,7 +698,7 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
         if (walk_wave_row<U>(wa, it) >= s_end) break;  // wave-uniform
         const uint32_t row = walk_row<U>(wa, it);
         Key<KW, NC> sk[U];
-        uint32_t sid[U], st[U];  // st: 16-bit tag << 16 | slot (slots < 2^16)
+        uint32_t sid[U], st[U];  // st: 16-bit tag << 16 | bucket (buckets < 2^16)
         uint32_t act = 0;        // bit u: row u of this lane still probing
         if (!PF && it > 0) walk_load<U, KW, NC>(wa, it, nk);
 #pragma unroll
@@ -713,34 +714,54 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const uint64_t h = hash_key<HP, KW, NC>(sk[u], hc);
-            st[u] = (((uint32_t)(h >> tag_shift) & 0xFFFFu) << 16) | ((uint32_t)(h >> shift) & smask);
+            st[u] = (((uint32_t)(h >> tag_shift) & 0xFFFFu) << 16) | ((uint32_t)(h >> shift) & bmask);
         }
+        // A round reads one whole bucket (kBucketSlots entries, one 16-byte LDS
+        // load) for every unfinished row; at load <= 1/2 a row nearly always
+        // ends in its first bucket.
         while (__ballot(act != 0)) {
-            uint32_t e[U];
+            uint4 E[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) e[u] = slots[st[u] & 0xFFFFu];  // always a valid slot
+            for (int u = 0; u < U; u++) E[u] = *reinterpret_cast<const uint4 *>(slots + ((st[u] & 0xFFFFu) << 2));
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const bool live = (act >> u) & 1u;
-                const bool hit_tag = e[u] != 0 && (e[u] >> 16) == (st[u] >> 16);
-                const uint32_t o = hit_tag ? (e[u] & 0xFFFFu) - 1 : 0u;
-                bool eq = hit_tag;
+                const uint32_t tg = st[u] >> 16;
+                const uint32_t e4[kBucketSlots] = {E[u].x, E[u].y, E[u].z, E[u].w};
+                uint32_t cand = 0;  // entries whose tag matches, in slot order
+                bool emp = false;   // the bucket has a free slot: the row's keys end here
 #pragma unroll
-                for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == sk[u].c[c];
-                const bool m = live && eq;
-                // a row is done at an empty slot, or at its (only) match
-                const bool stop = e[u] == 0 || (eq && !wa.dup);
-                act &= ~((live && stop ? 1u : 0u) << u);
-                st[u] = (st[u] & 0xFFFF0000u) | ((st[u] + 1) & smask);
-                if (MODE == WALK_EMIT) {
-                    em.put(m, rid[o] + wa.ob1, sid[u] + wa.ob1, wa);
-                } else {
-                    if (MODE == WALK_RECORD) {
-                        const uint32_t cur = res_get(res, it * U + u);
-                        res_set(res, it * U + u, m ? o : cur);
-                    }
-                    total += __builtin_popcountll(__ballot(m));
+                for (int j = 0; j < kBucketSlots; j++) {
+                    cand |= (e4[j] != 0 && (e4[j] >> 16) == tg ? 1u : 0u) << j;
+                    emp |= e4[j] == 0;
                 }
+                if (!live) cand = 0;
+                bool done = false;  // matched, no duplicates: the row is finished
+                while (__ballot(cand != 0)) {  // nearly always one pass
+                    const bool has = cand != 0;
+                    const uint32_t j = __builtin_ctz(cand | (1u << kBucketSlots));
+                    const uint32_t ej = j == 0 ? e4[0] : (j == 1 ? e4[1] : (j == 2 ? e4[2] : e4[3]));
+                    cand &= cand - 1;
+                    const uint32_t o = has ? (ej & 0xFFFFu) - 1 : 0u;
+                    bool eq = has;
+#pragma unroll
+                    for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == sk[u].c[c];
+                    if (MODE == WALK_EMIT) {
+                        em.put(eq, rid[o] + wa.ob1, sid[u] + wa.ob1, wa);
+                    } else {
+                        if (MODE == WALK_RECORD) {
+                            const uint32_t cur = res_get(res, it * U + u);
+                            res_set(res, it * U + u, eq ? o : cur);
+                        }
+                        total += __builtin_popcountll(__ballot(eq));
+                    }
+                    if (eq && !wa.dup) {
+                        done = true;
+                        cand = 0;
+                    }
+                }
+                act &= ~((live && (done || emp) ? 1u : 0u) << u);
+                st[u] = (st[u] & 0xFFFF0000u) | ((st[u] + 1) & bmask);
             }
         }
     }
@@ -875,8 +896,8 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
         if (t.done) break;
         if (w == 0 && lane == 0) next_task = atomicAdd(&a.ctl->task_ticket, 1ull);
 
-        // build: load the R chunk, then insert (index + 1, 16-bit tag) with linear probing
-        const uint32_t r_lo = t.r_lo, r_n = t.r_n, nslots = t.nslots, smask = nslots - 1, shift = t.shift;
+        // build: load the R chunk, then insert (index + 1, 16-bit tag) into buckets
+        const uint32_t r_lo = t.r_lo, r_n = t.r_n, nslots = t.nslots, bmask = (nslots >> 2) - 1, shift = t.shift;
         for (uint32_t i = tid; i < nslots; i += kProbeThreads) slots[i] = 0;
         for (uint32_t i = tid; i < r_n; i += kProbeThreads) {
             Key<KW, NC> k = load_key<KW, NC>(t.r, r_lo + i);
@@ -892,18 +913,24 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
             const uint64_t h = hash_key<HP, KW, NC>(k, hc);
             const uint32_t tag = (uint32_t)(h >> tag_shift) & 0xFFFFu;
             const uint32_t e = (tag << 16) | (i + 1);
-            uint32_t sl = (uint32_t)(h >> shift) & smask;
+            uint32_t b = (uint32_t)(h >> shift) & bmask;
+            // first free slot of the bucket, else the next bucket (linear probing
+            // over buckets); an equal key met on the way marks the chunk dup
             while (true) {
-                const uint32_t old = atomicCAS(&slots[sl], 0u, e);
-                if (old == 0) break;
-                if ((old >> 16) == tag) {
-                    const uint32_t o = (old & 0xFFFFu) - 1;
-                    bool eq = true;
+                uint32_t j = 0;
+                for (; j < (uint32_t)kBucketSlots; j++) {
+                    const uint32_t old = atomicCAS(&slots[b * kBucketSlots + j], 0u, e);
+                    if (old == 0) break;
+                    if ((old >> 16) == tag) {
+                        const uint32_t o = (old & 0xFFFFu) - 1;
+                        bool eq = true;
 #pragma unroll
-                    for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == k.c[c];
-                    if (eq) t.dup = 1;
+                        for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == k.c[c];
+                        if (eq) t.dup = 1;
+                    }
                 }
-                sl = (sl + 1) & smask;
+                if (j < (uint32_t)kBucketSlots) break;
+                b = (b + 1) & bmask;
             }
         }
         __syncthreads();

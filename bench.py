@@ -194,8 +194,17 @@ def batched_leg(ctx, n_joins, reps, seed=None):
         outs_r.append(torch.empty(cap, dtype=torch.int32, device="cuda"))
         outs_s.append(torch.empty(cap, dtype=torch.int32, device="cuda"))
         caps.append(cap)
-    del joins
     opts = vjoin.join_opts("fnv", 64, 64)
+    # autotune the radix plan of every build-size class present (one sample join
+    # each); the batched and the plain runs below both use the tuned plans
+    t0 = time.perf_counter()
+    tuned = {}
+    for i, (rcols, _) in enumerate(joins):
+        cls = max(len(rcols[0]) - 1, 1).bit_length()
+        if cls not in tuned:
+            tuned[cls] = ctx.tune(builds[i], probes[i], opts, reps=2)[0]
+    t_tune = time.perf_counter() - t0
+    del joins
     torch.cuda.synchronize()
 
     def batch(graph):
@@ -241,6 +250,7 @@ def batched_leg(ctx, n_joins, reps, seed=None):
             "joins_per_s_batched_graph": n_joins / tb, "joins_per_s_batched_eager": n_joins / te,
             "joins_per_s_plain": n_joins / tp, "speedup_vs_plain": tp / tb,
             "ms_batched_graph": tb * 1e3, "ms_batched_eager": te * 1e3, "ms_plain": tp * 1e3,
+            "autotuned_plans": {"2^%d" % k: list(v) for k, v in sorted(tuned.items())}, "tuning_s": t_tune,
             "same_counts_and_checksums": bool(same)}
 
 
@@ -435,8 +445,12 @@ def main():
     del rcols, scols
     ctx = vjoin.Context(0)
     ctx.set_profiling(True)
-    opts = vjoin.join_opts("fnv", 64, 64, passes=args.passes)
     build, probe = vjoin.table(dr), vjoin.table(ds)
+    # radix bits autotuned for the configured pass count (vj_tune, on these tables)
+    (tp, tb1, tb2), tune_ms = ctx.tune(build, probe, vjoin.join_opts("fnv", 64, 64, passes=args.passes), reps=3)
+    opts = vjoin.join_opts("fnv", 64, 64, passes=tp, bits1=tb1, bits2=tb2)
+    print("bench: autotuned plan passes=%d bits=%d+%d (%.3f ms)" % (tp, tb1, tb2, tune_ms), file=sys.stderr,
+          flush=True)
     t_alloc = time.perf_counter()
     out_r = torch.empty(ns, dtype=torch.int32, device="cuda")
     out_s = torch.empty(ns, dtype=torch.int32, device="cuda")
@@ -502,8 +516,8 @@ def main():
         "dtype": "int64",
         "data": "synthetic (seeded: workloads.large_fk)",
         "config": {"workload": "large_fk: build 2^%d unique int64 keys (mix64 of a permutation), probe 2^%d FK "
-                               "keys (100%% match), FNV-1a-64, %d radix passes" % (args.log_r, args.log_s,
-                                                                                   summ["passes"]),
+                               "keys (100%% match), FNV-1a-64, %d radix passes, bits autotuned" % (
+                                  args.log_r, args.log_s, summ["passes"]),
                    "build_rows": nr, "probe_rows": ns, "matches": m, "passes": summ["passes"],
                    "bits": [summ["bits1"], summ["bits2"]], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "%s (%s)" % (STAGE_KERNEL[dominant], dominant),

@@ -224,6 +224,24 @@ vj_status vj_join_batch(vj_ctx *ctx, int32_t n_joins, const vj_table *builds, co
                         const vj_join_opts *opts, uint32_t *const *out_build, uint32_t *const *out_probe,
                         const uint64_t *caps, uint64_t *counts, int32_t use_graph, void *stream);
 
+/*
+ * Autotuner (no counterpart in the reference, whose radix width per level is
+ * a configuration value, join_provider.cuh:296-300). Times every candidate
+ * plan near the planner's choice (each pass count; radix bits -1..+1; both
+ * splits of two passes) on the caller's tables, `reps` timed runs each after
+ * a warm-up, and keeps the fastest in the context for the size class of
+ * `build` (build rows rounded up to a power of two, key bytes, hash path).
+ * Later joins on this context with opts->passes == -1 and no bits set use the
+ * tuned plan for their size class (vj_join, vj_join_into, vj_join_batch).
+ * With opts->passes >= 0 only plans of that pass count are tried.
+ * *best receives opts with the chosen passes/bits, *best_ms its median time.
+ * Allocates a temporary output; synchronous.
+ */
+vj_status vj_tune(vj_ctx *ctx, const vj_table *build, const vj_table *probe, const vj_join_opts *opts, int32_t reps,
+                  vj_join_opts *best, double *best_ms, void *stream);
+/* Forgets every tuned plan of the context. */
+vj_status vj_tune_clear(vj_ctx *ctx);
+
 /* ------------------------------------------------------------------------ */
 /* Resident build sides. No counterpart in the reference, whose partitioned  */
 /* tables live for one join (join_gpu, join_provider.cuh:286-421): a served  */

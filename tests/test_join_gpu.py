@@ -269,6 +269,29 @@ def test_batch_graph_replays_match_eager(ctx):
     run(ia, want_a, False, "eager a after graphs")
 
 
+def test_tune_keeps_fastest_plan_and_results():
+    """vj_tune picks a plan among the candidates around the planner's choice,
+    later planner-driven joins of the size class use it, and results match."""
+    rcols, scols = workloads.plain_small(log_r=14, log_s=17)
+    dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+    want = np_join(workloads.reference_rows(rcols), workloads.reference_rows(scols))
+    c = vjoin.Context(0)
+    try:
+        c.set_profiling(True)
+        (P, b1, b2), ms = c.tune(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts(), reps=2)
+        assert ms > 0 and P in (0, 1, 2)
+        assert (P == 0 and b1 == 0) or (P >= 1 and 1 <= b1 <= 8) and (P < 2 or 1 <= b2 <= 8)
+        r, s_ = c.join(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts())
+        sm = c.summary()
+        assert (sm["passes"], sm["bits1"], sm["bits2"] if P == 2 else 0) == (P, b1, b2 if P == 2 else 0)
+        _assert_same(_pairs(r, s_), want, "tuned")
+        c.tune_clear()
+        r, s_ = c.join(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts())
+        _assert_same(_pairs(r, s_), want, "untuned")
+    finally:
+        c.close()
+
+
 def test_budget_admission(ctx):
     rcols, scols = workloads.plain_small(log_r=14, log_s=18)
     dr, ds = workloads.to_device(rcols), workloads.to_device(scols)

@@ -9,7 +9,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "vj_host.hpp"
@@ -51,6 +53,8 @@ struct vj_ctx {
     hipGraphExec_t graph = nullptr;
     std::vector<uint64_t> graph_sig;
     uint64_t graph_replays = 0, graph_captures = 0;
+    // Autotuned radix plans (vj_tune): size class -> passes | bits1 << 8 | bits2 << 16
+    std::unordered_map<uint64_t, uint32_t> tuned;
 };
 
 namespace {
@@ -312,11 +316,18 @@ void finish_layout(Plan &pl) {
     pl.meta_bytes = o;
 }
 
+// Size class of an autotuned plan: build rows (rounded up to a power of two),
+// key bytes and hash path.
+uint64_t tune_key(uint64_t nr, int kb, int hp) {
+    return (uint64_t)ceil_log2(std::max<uint64_t>(nr, 1)) | (uint64_t)kb << 8 | (uint64_t)hp << 16;
+}
+
 bool same_hash(const vj_hash_opts &a, const vj_hash_opts &b) {
     return a.kind == b.kind && a.hash_bits == b.hash_bits && a.chunk_bits == b.chunk_bits && a.word_mode == b.word_mode;
 }
 
-Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_opts *o) {
+Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_opts *o,
+                    const vj_ctx *c = nullptr) {
     const vj_table *t0 = specs[0].s;
     VJ_TRY(check_table(t0));
     VJ_TRY(make_hash_cfg(&o->hash, t0, &pl.hash));
@@ -344,6 +355,14 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
         } else {
             VJ_TRY(check_pair(js.r, js.s));
             VJ_TRY(plan_bits(js.r->n_rows, pl.kb, o, &P, &b1, &b2));
+            if (c && o->passes == -1 && !o->bits1 && !o->bits2 && !c->tuned.empty()) {  // autotuned plan
+                auto it = c->tuned.find(tune_key(js.r->n_rows, pl.kb, pl.shape.hp));
+                if (it != c->tuned.end()) {
+                    P = (int)(it->second & 0xFF);
+                    b1 = (int)((it->second >> 8) & 0xFF);
+                    b2 = (int)((it->second >> 16) & 0xFF);
+                }
+            }
             if (js.keep && P == 0) {  // a kept table is always partitioned
                 P = 1;
                 b1 = 1;
@@ -636,7 +655,7 @@ Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hi
               std::vector<uint64_t> &counts, bool grow, uint64_t extra_bytes, int *retries_out,
               bool use_graph = false) {
     Plan pl;
-    VJ_TRY(build_join_plan(pl, specs, o));
+    VJ_TRY(build_join_plan(pl, specs, o, c));
     VJ_TRY(admit(c, pl, extra_bytes + pl.keep_bytes));
     VJ_TRY(allocate(c, pl));
     PipeArgs a = make_args(c, pl);
@@ -717,7 +736,7 @@ Err pipeline_launch(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts 
                     size_t *count_off) {
     VJ_TRY(set_device(c));
     Plan pl;
-    VJ_TRY(build_join_plan(pl, specs, o));
+    VJ_TRY(build_join_plan(pl, specs, o, c));
     c->budget = budget;
     VJ_TRY(admit(c, pl, pl.keep_bytes));
     VJ_TRY(allocate(c, pl));
@@ -1060,6 +1079,103 @@ vj_status vj_built_free(vj_built *b) {
     (void)hipSetDevice(b->device);
     free_built(b);
     delete b;
+    return VJ_OK;
+}
+
+vj_status vj_tune(vj_ctx *c, const vj_table *build, const vj_table *probe, const vj_join_opts *opts, int32_t reps,
+                  vj_join_opts *best, double *best_ms, void *stream) {
+    if (!c || !best || reps < 1) return VJ_ERR_INVALID_ARGUMENT;
+    c->last_error.clear();
+    vj_join_opts d;
+    vj_join_opts_default(&d);
+    if (!opts) opts = &d;
+    Err e = check_pair(build, probe);
+    if (e.failed()) return fail(c, e);
+    e = set_device(c);
+    if (e.failed()) return fail(c, e);
+    hipStream_t s = (hipStream_t)stream;
+    HashCfg hc;
+    e = make_hash_cfg(&opts->hash, build, &hc);
+    if (e.failed()) return fail(c, e);
+    const int kb = build->key_width * build->n_cols, hp = hash_path(hc);
+    const uint64_t nr = build->n_rows;
+    // candidates around the planner's choice: every pass count, bits -1..+1
+    const uint64_t target = build_cap(kb) / 2;
+    const int need = nr > target ? ceil_log2((nr + target - 1) / target) : 0;
+    std::vector<std::array<int, 3>> cands;
+    if (nr <= (uint64_t)build_cap(kb)) cands.push_back({0, 0, 0});
+    for (int b = std::max(1, need - 1); b <= std::min(kMaxPassBits, need + 1); b++) cands.push_back({1, b, 0});
+    for (int tot = std::max(2, need - 1); tot <= std::min(kMaxHistBits, need + 1); tot++) {
+        for (int b1 : {(tot + 1) / 2, tot / 2}) {
+            const int b2 = tot - b1;
+            if (b1 < 1 || b1 > kMaxPassBits || b2 < 1 || b2 > kMaxPassBits) continue;
+            bool seen = false;
+            for (auto &x : cands) seen |= x[0] == 2 && x[1] == b1 && x[2] == b2;
+            if (!seen) cands.push_back({2, b1, b2});
+        }
+    }
+    if (opts->passes >= 0) {  // a fixed pass count: tune the bits only
+        std::vector<std::array<int, 3>> keep;
+        for (auto &x : cands)
+            if (x[0] == opts->passes) keep.push_back(x);
+        if (keep.empty()) return fail(c, Err::make(VJ_ERR_UNSUPPORTED, "no candidate plan with that pass count"));
+        cands.swap(keep);
+    }
+    // output buffers sized by a counting run (the count does not depend on the plan)
+    std::vector<JoinSpec> specs{{build, probe, nullptr, nullptr, 0}};
+    std::vector<uint64_t> counts;
+    c->tuned.erase(tune_key(nr, kb, hp));
+    vj_join_opts oc = *opts;  // counting run: any valid plan
+    oc.passes = cands[0][0];
+    oc.bits1 = cands[0][1];
+    oc.bits2 = cands[0][2];
+    e = run_joins(c, specs, &oc, s, counts, false, 0, nullptr);
+    if (e.failed()) return fail(c, e);
+    const uint64_t cap = std::max<uint64_t>(counts[0], 1);
+    uint32_t *ob = nullptr, *op = nullptr;
+    if (hipMalloc(&ob, cap * 4) != hipSuccess || hipMalloc(&op, cap * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        if (ob) (void)hipFree(ob);
+        return fail(c, Err::make(VJ_ERR_OUT_OF_MEMORY, "tuning output allocation failed"));
+    }
+    specs[0].out_r = ob;
+    specs[0].out_s = op;
+    specs[0].cap = cap;
+    double best_t = 1e30;
+    std::array<int, 3> best_c = cands[0];
+    for (auto &cand : cands) {
+        vj_join_opts o2 = *opts;
+        o2.passes = cand[0];
+        o2.bits1 = cand[1];
+        o2.bits2 = cand[2];
+        std::vector<double> ts;
+        for (int r = 0; r <= reps && !e.failed(); r++) {  // the first run warms the arenas
+            e = run_joins(c, specs, &o2, s, counts, false, 0, nullptr);
+            if (r) ts.push_back(c->summary.ms_total);
+        }
+        if (e.failed()) break;
+        std::sort(ts.begin(), ts.end());
+        const double med = ts[ts.size() / 2];
+        if (med < best_t) {
+            best_t = med;
+            best_c = cand;
+        }
+    }
+    (void)hipFree(ob);
+    (void)hipFree(op);
+    if (e.failed()) return fail(c, e);
+    c->tuned[tune_key(nr, kb, hp)] = (uint32_t)best_c[0] | (uint32_t)best_c[1] << 8 | (uint32_t)best_c[2] << 16;
+    *best = *opts;
+    best->passes = best_c[0];
+    best->bits1 = best_c[1];
+    best->bits2 = best_c[2];
+    if (best_ms) *best_ms = best_t;
+    return VJ_OK;
+}
+
+vj_status vj_tune_clear(vj_ctx *c) {
+    if (!c) return VJ_ERR_INVALID_ARGUMENT;
+    c->tuned.clear();
     return VJ_OK;
 }
 

@@ -113,6 +113,8 @@ def lib():
             "vj_partition": ([vp, P(Table), P(HashOpts), i32, i32, P(Table), vp, vp, vp], i32),
             "vj_join_batch": ([vp, i32, P(Table), P(Table), P(JoinOpts), P(vp), P(vp), P(u64), P(u64), i32, vp],
                               i32),
+            "vj_tune": ([vp, P(Table), P(Table), P(JoinOpts), i32, P(JoinOpts), P(ctypes.c_double), vp], i32),
+            "vj_tune_clear": ([vp], i32),
             "vj_build": ([vp, P(Table), P(JoinOpts), P(vp), vp], i32),
             "vj_probe_into": ([vp, vp, P(Table), P(JoinOpts), vp, vp, u64, P(u64), vp], i32),
             "vj_built_bytes": ([vp, P(u64)], i32),
@@ -265,6 +267,17 @@ class Context:
         return lib().vj_partition(self.ptr, ctypes.byref(tab), ctypes.byref(hopts), shift, bits,
                                   ctypes.byref(out_tab), ctypes.c_void_p(hist.data_ptr()),
                                   ctypes.c_void_p(offsets.data_ptr()), stream_ptr())
+
+    def tune(self, build, probe, opts, reps=3):
+        """vj_tune: returns ((passes, bits1, bits2), median ms) of the fastest plan, kept for the size class."""
+        best = JoinOpts()
+        ms = ctypes.c_double()
+        check(lib().vj_tune(self.ptr, ctypes.byref(build), ctypes.byref(probe), ctypes.byref(opts), int(reps),
+                            ctypes.byref(best), ctypes.byref(ms), stream_ptr()), "vj_tune", self.ptr)
+        return (best.passes, best.bits1, best.bits2), ms.value
+
+    def tune_clear(self):
+        check(lib().vj_tune_clear(self.ptr), "vj_tune_clear", self.ptr)
 
     def build(self, table, opts):
         """vj_build: a resident, partitioned build side (Built)."""

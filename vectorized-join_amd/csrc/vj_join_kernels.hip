@@ -395,10 +395,15 @@ struct PassTile {
     uint32_t valid;
 };
 
+// Waves per SIMD the pass kernel is compiled for: 8 (64 VGPRs, four blocks per
+// CU) for single 8-byte keys with FNV / XOR hashing; wider keys and the
+// generic hash path get 6 (80 VGPRs, three blocks) to limit spills.
+template <int HP, int KB> constexpr int pass_waves_per_simd() { return KB <= 8 && HP != HP_GENERIC ? 8 : 6; }
+
 // Pass 1: one tile of a job. Pass 2: one (pass-1 digit, segment) chunk, in
 // tile-sized steps; a running count per digit carries the destination.
 template <int HP, int KW, int NC>
-__global__ __launch_bounds__(kPassThreads, 6) void k_pass(PipeArgs a, int pass) {
+__global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>())) void k_pass(PipeArgs a, int pass) {
     using KT = typename KeyT<KW>::T;
     constexpr int KB = KW * NC;
     constexpr int ITEMS = pass_items(KB);

@@ -41,7 +41,7 @@ constexpr int kMaxHistBits = 14;    // 2D histogram in LDS: 2^14 x 4 B = 64 KiB
 constexpr uint32_t kXcds = 8;       // accelerator dies (XCDs) of an MI355X
 
 // Tuples per pass tile, per build chunk and per probe chunk, by key bytes per row.
-__host__ __device__ constexpr int pass_items(int key_bytes) { return key_bytes <= 8 ? 8 : (key_bytes <= 16 ? 4 : 2); }
+__host__ __device__ constexpr int pass_items(int key_bytes) { return key_bytes <= 8 ? 6 : (key_bytes <= 16 ? 4 : 2); }
 __host__ __device__ constexpr int pass_tile(int key_bytes) { return kPassThreads * pass_items(key_bytes); }
 __host__ __device__ constexpr int build_cap(int key_bytes) { return key_bytes <= 8 ? 2048 : 1024; }
 constexpr int kProbeChunk = 16384;  // probe rows per build+probe task

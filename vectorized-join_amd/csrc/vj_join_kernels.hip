@@ -586,13 +586,15 @@ struct ProbeTask {
     uint32_t wave_count[kProbeWaves];
 };
 
-constexpr int kProbeUnroll = 4;
+constexpr int kProbeUnroll = 2;
 constexpr int kBucketSlots = 4;  // LDS table: buckets of 4 slots (one 16-byte load)
 constexpr uint16_t kNoMatch = 0xFFFF;
 
 // One task's parameters for the walks (all wave-uniform).
 struct WalkArgs {
     TView s;
+    const uint32_t *r_ids;  // build rows' ids (null: the row index)
+    uint32_t r_lo;
     uint32_t *out_r, *out_s;
     uint64_t cap;
     uint32_t s_lo, s_end, bmask, shift, ob1;
@@ -604,6 +606,8 @@ __device__ __forceinline__ WalkArgs walk_args(const ProbeTask &t) {
     for (int c = 0; c < 4; c++) w.s.cols[c] = uniform(t.s.cols[c]);
     w.s.rowids = uniform(t.s.rowids);
     w.s.n = 0;
+    w.r_ids = uniform(t.r.rowids);
+    w.r_lo = uniform(t.r_lo);
     w.out_r = uniform(t.out_r);
     w.out_s = uniform(t.out_s);
     w.cap = uniform((uint64_t)t.cap);
@@ -614,6 +618,12 @@ __device__ __forceinline__ WalkArgs walk_args(const ProbeTask &t) {
     w.ob1 = uniform(t.one_based);
     w.dup = uniform(t.dup) != 0;
     return w;
+}
+
+// Row id of build row o of the task's chunk, read from the partitioned build
+// side (its rows were just loaded by the build step, so the read is cached).
+__device__ __forceinline__ uint32_t build_id(const WalkArgs &wa, uint32_t o) {
+    return wa.r_ids ? wa.r_ids[wa.r_lo + o] : wa.r_lo + o;
 }
 
 // Writes a wave's matches straight to the join output: the matching lanes of
@@ -684,7 +694,7 @@ __device__ __forceinline__ void walk_load(const WalkArgs &wa, int it, Key<KW, NC
 
 template <int MODE, int HP, int KW, int NC>
 __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg &hc, uint32_t tag_shift,
-                                               const typename KeyT<KW>::T *rk, const uint32_t *rid,
+                                               const typename KeyT<KW>::T *rk,
                                                const uint32_t *slots, uint32_t (&res)[kProbeIters * kProbeUnroll / 2],
                                                Emitter &em) {
     constexpr int CAP = build_cap(KW * NC);
@@ -752,7 +762,7 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
 #pragma unroll
                     for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == sk[u].c[c];
                     if (MODE == WALK_EMIT) {
-                        em.put(eq, rid[o] + wa.ob1, sid[u] + wa.ob1, wa);
+                        em.put(eq, build_id(wa, o) + wa.ob1, sid[u] + wa.ob1, wa);
                     } else {
                         if (MODE == WALK_RECORD) {
                             const uint32_t cur = res_get(res, it * U + u);
@@ -773,10 +783,11 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
     return total;
 }
 
-// Writes the matches a WALK_RECORD walk kept. The S row ids of a group of G
-// iterations are loaded before any is written, so their round trips overlap.
+// Writes the matches a WALK_RECORD walk kept. The probe-side and build-side
+// row ids of a group of G iterations are loaded before any is written, so
+// their round trips overlap.
 template <int G>
-__device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint32_t *rid,
+__device__ __forceinline__ void replay_walk(const WalkArgs &wa,
                                             const uint32_t (&res)[kProbeIters * kProbeUnroll / 2], Emitter &em) {
     constexpr int U = kProbeUnroll;
     static_assert(kProbeIters % G == 0, "replay groups");
@@ -784,16 +795,21 @@ __device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint32_t *
 #pragma unroll
     for (int g = 0; g < kProbeIters; g += G) {
         if (walk_wave_row<U>(wa, g) >= s_end) break;  // wave-uniform
-        uint32_t sid[G * U];
+        uint32_t sid[G * U], bid[G * U];
 #pragma unroll
         for (int it = 0; it < G; it++) {
             const uint32_t row = walk_row<U>(wa, g + it);
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const uint32_t i = row + u * 64;
+                const uint32_t o = res_get(res, (g + it) * U + u);
+                const bool m = i < s_end && o != kNoMatch;
                 sid[it * U + u] = i;
-                if (wa.s.rowids && i < s_end && res_get(res, (g + it) * U + u) != kNoMatch)
-                    sid[it * U + u] = wa.s.rowids[i];
+                bid[it * U + u] = 0;
+                if (m) {
+                    if (wa.s.rowids) sid[it * U + u] = wa.s.rowids[i];
+                    bid[it * U + u] = build_id(wa, o);
+                }
             }
         }
 #pragma unroll
@@ -801,9 +817,8 @@ __device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint32_t *
             if (walk_wave_row<U>(wa, g + it) >= s_end) break;  // wave-uniform
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const uint32_t o = res_get(res, (g + it) * U + u);
-                const bool m = o != kNoMatch;
-                em.put(m, rid[m ? o : 0u] + ob1, sid[it * U + u] + ob1, wa);
+                const bool m = res_get(res, (g + it) * U + u) != kNoMatch;
+                em.put(m, bid[it * U + u] + ob1, sid[it * U + u] + ob1, wa);
             }
         }
     }
@@ -812,7 +827,7 @@ __device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint32_t *
 // Waves per SIMD the probe kernel is compiled for: 6 (80 VGPRs, three blocks
 // per CU) for single 8-byte keys with FNV / XOR hashing; wider keys and the
 // generic hash path get 4 (128 VGPRs) so they do not spill.
-template <int HP, int KB> constexpr int probe_waves_per_simd() { return KB <= 8 && HP != HP_GENERIC ? 6 : 4; }
+template <int HP, int KB> constexpr int probe_waves_per_simd() { return KB <= 8 && HP != HP_GENERIC ? 8 : 4; }
 
 template <int HP, int KW, int NC>
 __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>())) void k_probe(PipeArgs a) {
@@ -821,8 +836,7 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
     constexpr int CAP = build_cap(KB);
     extern __shared__ __align__(16) unsigned char smem[];
     KT *rk = reinterpret_cast<KT *>(smem);                       // [NC][CAP]
-    uint32_t *rid = reinterpret_cast<uint32_t *>(rk + NC * CAP);  // [CAP]
-    uint32_t *slots = rid + CAP;                                  // [2 * CAP]
+    uint32_t *slots = reinterpret_cast<uint32_t *>(rk + NC * CAP);  // [2 * CAP]
     __shared__ ProbeTask t;
     uint32_t res[kProbeIters * kProbeUnroll / 2];  // matched build row per probe row, two per word
 
@@ -908,7 +922,6 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
             Key<KW, NC> k = load_key<KW, NC>(t.r, r_lo + i);
 #pragma unroll
             for (int c = 0; c < NC; c++) rk[c * CAP + i] = k.c[c];
-            rid[i] = t.r.rowids ? t.r.rowids[r_lo + i] : r_lo + i;
         }
         __syncthreads();
         for (uint32_t i = tid; i < r_n; i += kProbeThreads) {
@@ -944,8 +957,8 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
         // the task's output rows with one atomic, then write them
         const WalkArgs wa = walk_args(t);
         const bool dup = wa.dup;
-        const uint32_t wc = dup ? probe_walk<WALK_COUNT, HP, KW, NC>(wa, hc, tag_shift, rk, rid, slots, res, em)
-                                : probe_walk<WALK_RECORD, HP, KW, NC>(wa, hc, tag_shift, rk, rid, slots, res, em);
+        const uint32_t wc = dup ? probe_walk<WALK_COUNT, HP, KW, NC>(wa, hc, tag_shift, rk, slots, res, em)
+                                : probe_walk<WALK_RECORD, HP, KW, NC>(wa, hc, tag_shift, rk, slots, res, em);
         if (lane == 0) t.wave_count[w] = wc;
         __syncthreads();
         if (tid == 0) {
@@ -958,10 +971,10 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
             em.pos = t.out_base;
             for (uint32_t k = 0; k < w; k++) em.pos += t.wave_count[k];
             if (dup) {
-                probe_walk<WALK_EMIT, HP, KW, NC>(wa, hc, tag_shift, rk, rid, slots, res, em);
+                probe_walk<WALK_EMIT, HP, KW, NC>(wa, hc, tag_shift, rk, slots, res, em);
             } else {
-                // row-id groups of four iterations where registers allow (single 8-byte key, FNV / XOR)
-                replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 4 : 1>(wa, rid, res, em);
+                // row-id groups of two iterations where registers allow (single 8-byte key, FNV / XOR)
+                replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 2 : 1>(wa, res, em);
             }
         }
         __syncthreads();
@@ -1000,7 +1013,7 @@ size_t pass_lds_bytes(int kw, int nc) {
 
 size_t probe_lds_bytes(int kw, int nc) {
     const int kb = kw * nc, cap = build_cap(kb);
-    return (size_t)cap * kb + (size_t)cap * 4 + (size_t)cap * 8;
+    return (size_t)cap * kb + (size_t)cap * 8;
 }
 
 }  // namespace

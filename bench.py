@@ -119,9 +119,9 @@ def traffic_from_pmc(fetch_csv, write_csv, workload):
     lines = ["# HBM traffic per launch, from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch),",
              "# each scaled by counter(1 GiB device copy) / 1 GiB. Workload: " + workload,
              "# made by: rocprofv3 --pmc FETCH_SIZE -d F -o run -- python3 bench.py --steps 1 --warmup 1 --no-cpu "
-             "--calib-copy",
+             "--no-batched --no-serving --no-tune --calib-copy",
              "#          rocprofv3 --pmc WRITE_SIZE -d W -o run -- python3 bench.py --steps 1 --warmup 1 --no-cpu "
-             "--calib-copy",
+             "--no-batched --no-serving --no-tune --calib-copy",
              "#          python3 bench.py --traffic-from F/run_results.db W/run_results.db"]
     factors, per = {}, {}
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
@@ -411,6 +411,7 @@ def main():
     ap.add_argument("--batched-joins", type=int, default=256)
     ap.add_argument("--batched-reps", type=int, default=3)
     ap.add_argument("--no-serving", action="store_true", help="skip the served-vs-plain request stream")
+    ap.add_argument("--no-tune", action="store_true", help="use the planner's radix bits instead of vj_tune")
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--serving-rates", type=float, nargs="+", default=[2000.0, 8000.0, 32000.0])
     ap.add_argument("--calib-copy", action="store_true",
@@ -446,11 +447,13 @@ def main():
     ctx = vjoin.Context(0)
     ctx.set_profiling(True)
     build, probe = vjoin.table(dr), vjoin.table(ds)
-    # radix bits autotuned for the configured pass count (vj_tune, on these tables)
-    (tp, tb1, tb2), tune_ms = ctx.tune(build, probe, vjoin.join_opts("fnv", 64, 64, passes=args.passes), reps=3)
-    opts = vjoin.join_opts("fnv", 64, 64, passes=tp, bits1=tb1, bits2=tb2)
-    print("bench: autotuned plan passes=%d bits=%d+%d (%.3f ms)" % (tp, tb1, tb2, tune_ms), file=sys.stderr,
-          flush=True)
+    if args.no_tune:  # planner bits (profiling runs: no extra joins in the trace)
+        opts = vjoin.join_opts("fnv", 64, 64, passes=args.passes)
+    else:  # radix bits autotuned for the configured pass count (vj_tune, on these tables)
+        (tp, tb1, tb2), tune_ms = ctx.tune(build, probe, vjoin.join_opts("fnv", 64, 64, passes=args.passes), reps=3)
+        opts = vjoin.join_opts("fnv", 64, 64, passes=tp, bits1=tb1, bits2=tb2)
+        print("bench: autotuned plan passes=%d bits=%d+%d (%.3f ms)" % (tp, tb1, tb2, tune_ms), file=sys.stderr,
+              flush=True)
     t_alloc = time.perf_counter()
     out_r = torch.empty(ns, dtype=torch.int32, device="cuda")
     out_s = torch.empty(ns, dtype=torch.int32, device="cuda")

@@ -141,6 +141,23 @@ template <typename T> __device__ void block_scan_array(const T *in, T *out, uint
     __syncthreads();
 }
 
+// Loads in[0..n) so that thread t holds its contiguous run
+// [t*per, t*per + per) in v[0..per) (per <= kScanChunk): coalesced global loads
+// into LDS (`lds` holds n + n/32 words, padded against bank conflicts), then
+// each thread reads its run. Also returns in[t*per + per] (the element after
+// the run, if < n) in *next. Block-wide; ends with a barrier.
+__host__ __device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 5); }
+__device__ void load_runs(const uint32_t *in, uint32_t n, uint32_t per, uint32_t (&v)[kScanChunk], uint32_t *next,
+                          uint32_t *lds) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[lds_pad(i)] = in[i];
+    __syncthreads();
+    const uint32_t lo = threadIdx.x * per;
+#pragma unroll
+    for (int j = 0; j < kScanChunk; j++) v[j] = (uint32_t)j < per && lo + j < n ? lds[lds_pad(lo + j)] : 0u;
+    *next = lo + per < n ? lds[lds_pad(lo + per)] : 0u;
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 
 __global__ void k_prepare(PipeArgs a) {
@@ -276,6 +293,7 @@ __global__ __launch_bounds__(1024) void k_colscan(PipeArgs a) {
 // column totals (the last row of m2, or of m1 for one pass).
 __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
     __shared__ uint32_t sh32[20];
+    extern __shared__ uint32_t lds_runs[];  // nb + nb/32 words
     const JobDesc &job = a.jobs[blockIdx.x];
     uint32_t *base = a.base + job.base_off;
     if (job.b1 == 0) {
@@ -294,7 +312,22 @@ __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
         for (uint32_t q = threadIdx.x; q <= nb; q += blockDim.x) base[q] = 0;
     } else {
         const uint32_t *tot = a.counts + (job.b2 ? job.m2_off : job.m1_off) + (uint64_t)job.nseg * nb;
-        block_scan_array<uint32_t>(tot, base, nb, sh32);
+        const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;  // <= kScanChunk (nb <= 2^14)
+        uint32_t v[kScanChunk], nx;
+        load_runs(tot, nb, per, v, &nx, lds_runs);
+        uint32_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < kScanChunk; j++) sum += v[j];
+        uint32_t total;
+        uint32_t pre = block_excl_scan<uint32_t>(sum, sh32, &total);
+        const uint32_t lo = threadIdx.x * per;
+#pragma unroll
+        for (int j = 0; j < kScanChunk; j++) {
+            if ((uint32_t)j < per && lo + j < nb) base[lo + j] = pre;
+            pre += v[j];
+        }
+        if (threadIdx.x == 0) base[nb] = total;
+        __syncthreads();
     }
     if (job.keep_base) {
         __syncthreads();
@@ -311,43 +344,32 @@ __device__ __forceinline__ unsigned long long part_tasks(uint32_t nr, uint32_t n
 
 __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
     __shared__ unsigned long long sh64[20];
+    extern __shared__ uint32_t lds_runs[];  // (nparts + 1) + (nparts + 1)/32 words
     const uint32_t jn = blockIdx.x;
     const JoinDesc &jd = a.joins[jn];
     const uint32_t *rb = a.base + a.jobs[jd.job_r].base_off;
     const uint32_t *sb = a.base + a.jobs[jd.job_s].base_off;
     unsigned long long *pt = a.ptask + jd.ptask_off;
-    const uint32_t n = jd.nparts, per = (n + blockDim.x - 1) / blockDim.x;
-    const uint32_t lo = threadIdx.x * per < n ? threadIdx.x * per : n;
-    const uint32_t hi = lo + per < n ? lo + per : n;
-    constexpr int C = kScanChunk;
+    const uint32_t n = jd.nparts, per = (n + blockDim.x - 1) / blockDim.x;  // <= kScanChunk (nparts <= 2^14)
+    uint32_t r[kScanChunk], q[kScanChunk], rn, qn;
+    load_runs(rb, n + 1, per, r, &rn, lds_runs);
+    load_runs(sb, n + 1, per, q, &qn, lds_runs);
+    const uint32_t lo = threadIdx.x * per;
+    unsigned long long t[kScanChunk];
     unsigned long long sum = 0;
-    for (uint32_t b = lo; b < hi; b += C) {
-        uint32_t r[C + 1], q[C + 1];
 #pragma unroll
-        for (int k = 0; k <= C; k++) {
-            r[k] = b + k <= hi ? rb[b + k] : 0u;
-            q[k] = b + k <= hi ? sb[b + k] : 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < C; k++)
-            if (b + k < hi) sum += part_tasks(r[k + 1] - r[k], q[k + 1] - q[k], jd);
+    for (int k = 0; k < kScanChunk; k++) {
+        const uint32_t r1 = k + 1 < kScanChunk && (uint32_t)(k + 1) < per ? r[k + 1] : rn;
+        const uint32_t q1 = k + 1 < kScanChunk && (uint32_t)(k + 1) < per ? q[k + 1] : qn;
+        t[k] = (uint32_t)k < per && lo + k < n ? part_tasks(r1 - r[k], q1 - q[k], jd) : 0ull;
+        sum += t[k];
     }
     unsigned long long total;
     unsigned long long pre = block_excl_scan<unsigned long long>(sum, sh64, &total);
-    for (uint32_t b = lo; b < hi; b += C) {
-        uint32_t r[C + 1], q[C + 1];
 #pragma unroll
-        for (int k = 0; k <= C; k++) {
-            r[k] = b + k <= hi ? rb[b + k] : 0u;
-            q[k] = b + k <= hi ? sb[b + k] : 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < C; k++) {
-            if (b + k < hi) {
-                pt[b + k] = pre;
-                pre += part_tasks(r[k + 1] - r[k], q[k + 1] - q[k], jd);
-            }
-        }
+    for (int k = 0; k < kScanChunk; k++) {
+        if ((uint32_t)k < per && lo + k < n) pt[lo + k] = pre;
+        pre += t[k];
     }
     if (threadIdx.x == 0) {
         pt[n] = total;
@@ -1038,10 +1060,11 @@ hipError_t launch_scan(const PipeArgs &a, hipStream_t s) {
         hipLaunchKernelGGL(k_colscan, dim3(a.n_colscan), dim3(1024), 0, s, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_bases, dim3(a.n_jobs), dim3(1024), 0, s, a);
+    const size_t runs_lds = (size_t)(lds_pad((1u << kMaxHistBits) + 1) + 1) * 4;  // load_runs of <= 2^14 + 1 words
+    hipLaunchKernelGGL(k_bases, dim3(a.n_jobs), dim3(1024), runs_lds, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.n_joins) {
-        hipLaunchKernelGGL(k_tasks, dim3(a.n_joins), dim3(1024), 0, s, a);
+        hipLaunchKernelGGL(k_tasks, dim3(a.n_joins), dim3(1024), runs_lds, s, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, s, a);

@@ -437,7 +437,8 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>()))
     uint32_t *wcnt = reinterpret_cast<uint32_t *>(smem + STAGE);           // [kPassWaves][256]
     uint32_t *loc = wcnt + kPassWaves * 256;                               // [256]
     uint32_t *gdst = loc + 256;                                            // [256] next destination row
-    uint8_t *sdig = reinterpret_cast<uint8_t *>(gdst + 256);               // [T]
+    uint32_t *gdo = gdst + 256;                                            // [256] gdst - loc of the current tile
+    uint8_t *sdig = reinterpret_cast<uint8_t *>(gdo + 256);                // [T]
     __shared__ PassTile ti;
     __shared__ uint32_t sh_scan[kPassWaves + 1];
 
@@ -550,7 +551,10 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>()))
         }
         uint32_t total;
         const uint32_t my_loc = block_excl_scan<uint32_t>(cnt, sh_scan, &total);
-        if (tid < bins) loc[tid] = my_loc;
+        if (tid < bins) {
+            loc[tid] = my_loc;
+            gdo[tid] = gdst[tid] - my_loc;  // output row of staged slot k = gdo[digit] + k
+        }
         __syncthreads();
 
         // 3. stage the key columns in LDS ordered by digit; consecutive slots of
@@ -571,8 +575,7 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>()))
         // 4. write the keys in slot order
         const uint32_t n = hi - lo;
         for (uint32_t k = tid; k < n; k += kPassThreads) {
-            const uint32_t d = sdig[k];
-            const uint32_t dst = gdst[d] + (k - loc[d]);
+            const uint32_t dst = gdo[sdig[k]] + k;
 #pragma unroll
             for (int c = 0; c < NC; c++) dcols[c][dst] = skeys[c * T + k];
         }
@@ -582,10 +585,7 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>()))
         for (int it = 0; it < ITEMS; it++)
             if (slot[it] != ~0u) sids[slot[it]] = ids[it];
         __syncthreads();
-        for (uint32_t k = tid; k < n; k += kPassThreads) {
-            const uint32_t d = sdig[k];
-            dids[gdst[d] + (k - loc[d])] = sids[k];
-        }
+        for (uint32_t k = tid; k < n; k += kPassThreads) dids[gdo[sdig[k]] + k] = sids[k];
         __syncthreads();
         if (tid < bins) gdst[tid] += cnt;  // next step of a pass-2 chunk
     }
@@ -1030,7 +1030,7 @@ template <typename F> hipError_t dispatch(const LaunchShape &sh, F &&f) {
 
 size_t pass_lds_bytes(int kw, int nc) {
     const int kb = kw * nc, T = pass_tile(kb);
-    return (size_t)T * (kb > 4 ? kb : 4) + (size_t)(kPassWaves * 256 + 512) * 4 + (size_t)T;
+    return (size_t)T * (kb > 4 ? kb : 4) + (size_t)(kPassWaves * 256 + 768) * 4 + (size_t)T;
 }
 
 size_t probe_lds_bytes(int kw, int nc) {

@@ -396,6 +396,54 @@ def serving_leg(n_requests, rates, budget_gb=64, n_pool=64, n_tables=8, n_reprob
     return results
 
 
+def multicol_leg(ctx, steps):
+    """Multi-column skewed join: 4 x uint32 key columns, build 2^22 rows (2^20
+    distinct keys, each 4 times), probe 2^25 rows (80 % Zipf(1.0) over the
+    distinct keys, 20 % outside them), with FNV-1a-64 and custom_xor_hw-64.
+    Checked by size-independent properties: the match count is 4 x the hitting
+    probe rows, every pair has equal keys in all 4 columns, every probe row
+    appears 0 or 4 times."""
+    import numpy as np
+    import torch
+
+    import vjoin
+    import workloads
+    rcols, scols = workloads.multicol_skew()
+    nr, ns = len(rcols[0]), len(scols[0])
+    n_hit = int(np.count_nonzero(scols[0] < np.uint32(0x80000000)))
+    dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+    del rcols, scols
+    m_want = 4 * n_hit
+    out_r = torch.empty(m_want, dtype=torch.int32, device="cuda")
+    out_s = torch.empty(m_want, dtype=torch.int32, device="cuda")
+    res = {"workload": "build 2^22 rows of 4 x uint32 keys (2^20 distinct, x4), probe 2^25 rows (80% Zipf(1.0), 20% "
+                       "misses), 2 radix passes", "build_rows": nr, "probe_rows": ns, "matches": m_want}
+    for kind in ("fnv", "custom_xor_hw"):
+        opts = vjoin.join_opts(kind, 64, 64, passes=2)
+        b, p = vjoin.table(dr), vjoin.table(ds)
+        ms, stages = [], {k: [] for k in ("ms_histogram", "ms_pass1", "ms_pass2", "ms_build_probe")}
+        for i in range(steps + 1):
+            rc, cnt = ctx.join_into(b, p, opts, out_r, out_s)
+            if rc != 0:
+                raise RuntimeError("join failed: %s %s" % (vjoin.STATUS.get(rc), ctx.last_error()))
+            if i:
+                sm = ctx.summary()
+                ms.append(sm["ms_total"])
+                for k in stages:
+                    stages[k].append(sm[k])
+        sid, rid = out_s.long(), out_r.long()
+        eq = all(bool(torch.equal(dr[c][rid], ds[c][sid])) for c in range(4))
+        per_row = torch.bincount(sid, minlength=ns)
+        ok = cnt == m_want and eq and int((per_row == 4).sum()) == n_hit and int((per_row != 0).sum()) == n_hit
+        del sid, rid, per_row
+        t = float(np.median(ms))
+        B = 5 * (nr + ns) * 20 + m_want * 8
+        res[kind] = {"ms": t, "stage_ms": {k[3:]: float(np.median(v)) for k, v in stages.items()},
+                     "probe_tuples_per_s": ns / (t * 1e-3), "algorithmic_bytes": B,
+                     "achieved_GBps": B / (t * 1e-3) / 1e9, "frac": B / (t * 1e-3) / HBM_PEAK, "parity": bool(ok)}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -412,6 +460,7 @@ def main():
     ap.add_argument("--batched-reps", type=int, default=3)
     ap.add_argument("--no-serving", action="store_true", help="skip the served-vs-plain request stream")
     ap.add_argument("--no-tune", action="store_true", help="use the planner's radix bits instead of vj_tune")
+    ap.add_argument("--no-multicol", action="store_true", help="skip the multi-column skewed join leg")
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--serving-rates", type=float, nargs="+", default=[2000.0, 8000.0, 32000.0])
     ap.add_argument("--calib-copy", action="store_true",
@@ -534,6 +583,9 @@ def main():
         "output_alloc_ms": t_alloc * 1e3,
         "parity": {"fk_every_probe_row_once_equal_keys": ok},
     }
+    if not args.no_multicol:
+        print("bench: multi-column skewed join leg", file=sys.stderr, flush=True)
+        out["multicol_skew"] = multicol_leg(ctx, 5)
     if not args.no_batched:
         del dr, ds, build, probe, out_r, out_s
         torch.cuda.empty_cache()

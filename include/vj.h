@@ -173,6 +173,8 @@ vj_status vj_join(vj_ctx *ctx, const vj_table *build, const vj_table *probe, con
  * first `capacity` pairs (in unspecified order) are written and
  * VJ_ERR_CAPACITY is returned. No allocation happens when the context's
  * scratch is already large enough (the case for repeated joins of a shape).
+ * When per-stage profiling is off, the second call of one launch shape
+ * captures the pipeline in a hipGraph and later calls of that shape replay it.
  */
 vj_status vj_join_into(vj_ctx *ctx, const vj_table *build, const vj_table *probe, const vj_join_opts *opts,
                        uint32_t *out_build, uint32_t *out_probe, uint64_t capacity, uint64_t *count, void *stream);
@@ -188,6 +190,8 @@ typedef struct vj_join_summary {
     double ms_total;     /* all stages, device time */
     double ms_histogram, ms_pass1, ms_pass2, ms_build_probe;
     uint64_t scratch_bytes;
+    int32_t graph_replay;  /* 1: the join ran as a replay of a captured hipGraph (same launch shape as before) */
+    int32_t reserved;
 } vj_join_summary;
 
 vj_status vj_join_summary_get(vj_ctx *ctx, vj_join_summary *summary);

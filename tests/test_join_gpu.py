@@ -161,6 +161,44 @@ def test_join_into_capacity(ctx):
     _assert_same(_pairs(out_r, out_s), want)
 
 
+def test_join_into_replays_graph_for_a_repeated_shape():
+    """The second join_into of one launch shape captures a hipGraph and later
+    ones replay it; with other work and synchronisations in between, every
+    replay equals the oracle, and a new shape runs eagerly again."""
+    import torch
+    rcols, scols = workloads.plain_small(log_r=13, log_s=16)
+    dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+    want = np_join(workloads.reference_rows(rcols), workloads.reference_rows(scols))
+    r2, s2 = workloads.plain_small(log_r=11, log_s=14, seed=5)
+    dr2, ds2 = workloads.to_device(r2), workloads.to_device(s2)
+    want2 = np_join(workloads.reference_rows(r2), workloads.reference_rows(s2))
+    c = vjoin.Context(0)
+    try:
+        c.set_profiling(False)  # replay is automatic without per-stage timing
+        opts = vjoin.join_opts(passes=2, bits1=4, bits2=3)
+        replays = []
+        for rep in range(5):
+            o_r = torch.full((len(scols[0]),), -1, dtype=torch.int32, device="cuda")
+            o_s = torch.full((len(scols[0]),), -1, dtype=torch.int32, device="cuda")
+            rc, n = c.join_into(vjoin.table(dr), vjoin.table(ds), opts, o_r, o_s)
+            assert rc == 0, c.last_error()
+            sm = c.summary()
+            replays.append(sm["graph_replay"])
+            assert sm["ms_total"] > 0
+            _assert_same(_pairs(o_r[:n], o_s[:n]), want, ("rep", rep))
+            x = torch.arange(1000, device="cuda").sum()  # other work between replays
+            torch.cuda.synchronize()
+            assert int(x) == 499500
+        assert replays[0] == 0 and all(replays[2:]), replays
+        o_r = torch.empty(len(s2[0]), dtype=torch.int32, device="cuda")
+        o_s = torch.empty(len(s2[0]), dtype=torch.int32, device="cuda")
+        rc, n = c.join_into(vjoin.table(dr2), vjoin.table(ds2), opts, o_r, o_s)
+        assert rc == 0 and c.summary()["graph_replay"] == 0
+        _assert_same(_pairs(o_r[:n], o_s[:n]), want2, "other shape")
+    finally:
+        c.close()
+
+
 def test_vj_join_grows_output_beyond_hint(ctx):
     # 1:n join whose output is 4x the probe side: vj_join reruns build+probe with an exact buffer
     rng = np.random.default_rng(9)

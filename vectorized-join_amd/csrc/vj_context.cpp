@@ -52,6 +52,8 @@ struct vj_ctx {
     hipStream_t capture_stream = nullptr;
     hipGraphExec_t graph = nullptr;
     std::vector<uint64_t> graph_sig;
+    std::vector<uint64_t> last_sig;  // signature of the last eager pipeline (automatic capture)
+    bool graph_failed = false;       // capture failed once: stay eager
     uint64_t graph_replays = 0, graph_captures = 0;
     // Autotuned radix plans (vj_tune): size class -> passes | bits1 << 8 | bits2 << 16
     std::unordered_map<uint64_t, uint32_t> tuned;
@@ -553,7 +555,7 @@ Err launch_stages(vj_ctx *c, const Plan &pl, const PipeArgs &a, hipStream_t s, b
 // Everything a captured pipeline bakes in: the kernel arguments (arena
 // pointers, totals, hash), which stages run, the probe grid, and the copies.
 std::vector<uint64_t> graph_signature(const vj_ctx *c, const Plan &pl, const PipeArgs &a) {
-    std::vector<uint64_t> sig((sizeof(PipeArgs) + 7) / 8 + 10, 0);
+    std::vector<uint64_t> sig((sizeof(PipeArgs) + 7) / 8 + 10, 0);  // zeroed: PipeArgs padding is zero too
     memcpy(sig.data(), &a, sizeof(PipeArgs));
     uint64_t *x = sig.data() + (sizeof(PipeArgs) + 7) / 8;
     x[0] = (uint64_t)pl.any_part | (uint64_t)pl.any_p2 << 1;
@@ -576,7 +578,9 @@ void drop_graph(vj_ctx *c) {
 }
 
 // Captures upload -> all stages -> count download on the context's capture
-// stream (capture needs a stream other than the null stream) and instantiates it.
+// stream (capture needs a stream other than the null stream) and instantiates
+// it. Events are recorded around the graph launch, not inside the graph
+// (event nodes inside a replayed graph gave stage times that did not add up).
 Err capture_pipeline(vj_ctx *c, const Plan &pl, const PipeArgs &a) {
     drop_graph(c);
     if (!c->capture_stream) VJ_HIP_TRY(hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking));
@@ -616,7 +620,7 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-void fill_summary(vj_ctx *c, const Plan &pl, const JoinSpec &js, uint64_t matches, int retries) {
+void fill_summary(vj_ctx *c, const Plan &pl, const JoinSpec &js, uint64_t matches, int retries, bool replay) {
     vj_join_summary &s = c->summary;
     memset(&s, 0, sizeof(s));
     s.build_rows = js.res ? js.res->n : js.r->n_rows;
@@ -636,6 +640,7 @@ void fill_summary(vj_ctx *c, const Plan &pl, const JoinSpec &js, uint64_t matche
         s.ms_build_probe = elapsed(c->ev[EV_P2], c->ev[EV_PROBE]);
     }
     s.scratch_bytes = held_bytes(c);
+    s.graph_replay = replay ? 1 : 0;
 }
 
 vj_status fail(vj_ctx *c, const Err &e) {
@@ -651,9 +656,13 @@ Err set_device(vj_ctx *c) {
 // Runs the pipeline for `specs`; counts[i] receives join i's match count.
 // When `grow` is set, joins whose output overflowed get a fresh buffer of the
 // exact size (hipMalloc, owned by the caller) and only build+probe is rerun.
+// Graph use of a pipeline: never, always (vj_join_batch with use_graph), or
+// automatically when the same launch signature comes twice in a row.
+enum GraphMode { GRAPH_OFF, GRAPH_ON, GRAPH_AUTO };
+
 Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hipStream_t s,
               std::vector<uint64_t> &counts, bool grow, uint64_t extra_bytes, int *retries_out,
-              bool use_graph = false) {
+              GraphMode gmode = GRAPH_OFF) {
     Plan pl;
     VJ_TRY(build_join_plan(pl, specs, o, c));
     VJ_TRY(admit(c, pl, extra_bytes + pl.keep_bytes));
@@ -661,14 +670,30 @@ Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hi
     PipeArgs a = make_args(c, pl);
     char *h = static_cast<char *>(c->pinned);
     unsigned long long *hc = reinterpret_cast<unsigned long long *>(h + pl.off_count);
-    if (use_graph) {
-        // the previous use of the pinned buffer has completed (every call ends synchronised)
-        stage_descriptors(c, pl);
+    bool replay = false;
+    // automatic replay only without per-stage timing (it needs events between the stages)
+    if (gmode == GRAPH_AUTO && c->profiling) gmode = GRAPH_OFF;
+    if (gmode != GRAPH_OFF && !c->graph_failed) {
         std::vector<uint64_t> sig = graph_signature(c, pl, a);
-        if (!c->graph || sig != c->graph_sig) {
-            VJ_TRY(capture_pipeline(c, pl, a));
-            c->graph_sig = std::move(sig);
+        if (c->graph && sig == c->graph_sig) {
+            replay = true;
+        } else if (gmode == GRAPH_ON || sig == c->last_sig) {
+            // the previous use of the pinned buffer has completed (every call ends synchronised)
+            stage_descriptors(c, pl);
+            const Err ce = capture_pipeline(c, pl, a);
+            if (ce.failed()) {
+                if (gmode == GRAPH_ON) return ce;
+                c->graph_failed = true;  // automatic use only: stay eager from now on
+            } else {
+                c->graph_sig = std::move(sig);
+                replay = true;
+            }
+        } else {
+            c->last_sig = std::move(sig);
         }
+    }
+    if (replay) {
+        stage_descriptors(c, pl);  // the graph's upload node reads them at launch
         VJ_HIP_TRY(hipEventRecord(c->ev[EV_START], s));
         VJ_HIP_TRY(hipGraphLaunch(c->graph, s));
         VJ_HIP_TRY(hipEventRecord(c->ev[EV_PROBE], s));
@@ -716,7 +741,7 @@ Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hi
         }
     }
     if (retries_out) *retries_out = retries;
-    if (specs.size() == 1) fill_summary(c, pl, specs[0], counts[0], retries);
+    if (specs.size() == 1) fill_summary(c, pl, specs[0], counts[0], retries, replay);
     return Err::ok();
 }
 
@@ -853,7 +878,7 @@ vj_status vj_join_into(vj_ctx *c, const vj_table *build, const vj_table *probe, 
     if (e.failed()) return fail(c, e);
     std::vector<JoinSpec> specs{{build, probe, out_build, out_probe, capacity}};
     std::vector<uint64_t> counts;
-    e = run_joins(c, specs, opts, (hipStream_t)stream, counts, false, 0, nullptr);
+    e = run_joins(c, specs, opts, (hipStream_t)stream, counts, false, 0, nullptr, GRAPH_AUTO);
     if (e.failed()) return fail(c, e);
     *count = counts[0];
     if (counts[0] > capacity)
@@ -971,7 +996,7 @@ vj_status vj_join_batch(vj_ctx *c, int32_t n_joins, const vj_table *builds, cons
             return fail(c, Err::make(VJ_ERR_INVALID_ARGUMENT, "output is NULL"));
     }
     std::vector<uint64_t> cnt;
-    e = run_joins(c, specs, opts, (hipStream_t)stream, cnt, false, 0, nullptr, use_graph != 0);
+    e = run_joins(c, specs, opts, (hipStream_t)stream, cnt, false, 0, nullptr, use_graph ? GRAPH_ON : GRAPH_OFF);
     if (e.failed()) return fail(c, e);
     bool over = false;
     for (int i = 0; i < n_joins; i++) {

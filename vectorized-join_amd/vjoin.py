@@ -76,7 +76,7 @@ class JoinSummary(ctypes.Structure):
                 ("retries", ctypes.c_int32), ("partitions", ctypes.c_uint64), ("tasks", ctypes.c_uint64),
                 ("ms_total", ctypes.c_double), ("ms_histogram", ctypes.c_double), ("ms_pass1", ctypes.c_double),
                 ("ms_pass2", ctypes.c_double), ("ms_build_probe", ctypes.c_double),
-                ("scratch_bytes", ctypes.c_uint64)]
+                ("scratch_bytes", ctypes.c_uint64), ("graph_replay", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 def lib():

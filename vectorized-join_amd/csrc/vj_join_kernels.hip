@@ -669,18 +669,15 @@ struct Emitter {
 // Walks over this wave's rows of the S chunk. The chunk holds up to
 // kProbeChunk = kProbeIters x kProbeThreads x kProbeUnroll rows; in iteration
 // `it` lane `l` of wave `w` owns rows it*kProbeThreads*U + w*64*U + u*64 + l.
-//   WALK_COUNT   probe, count matches                          (build chunk has duplicate keys)
-//   WALK_EMIT    probe again, write the matches                (duplicates)
-//   WALK_RECORD  probe, count, keep each row's match in registers (no duplicates: <= 1 match)
-// replay_walk then writes the kept matches without probing. All walks visit
-// rows in the same order, so the counts of the first walk are the row counts
-// of the second. Probing runs in rounds: each round reads one table slot for
-// every unfinished row of the lane, so U independent LDS chains are in flight
-// per lane; a round is branch-free (selects, no per-row branches). The S keys
-// of the next iteration are loaded before the current one is probed, so one
-// HBM round trip overlaps each iteration's probing (where registers allow,
-// see PF).
-enum { WALK_COUNT = 0, WALK_EMIT = 1, WALK_RECORD = 2 };
+// The table holds one entry per distinct key (equal keys of the build chunk
+// are chained behind it, see the build step), so a row matches at most one
+// entry. probe_walk finds it, keeps its index in registers and counts the
+// row's matches (the chain length); replay_walk then writes them without
+// probing again. Probing runs in rounds: each round reads one whole bucket
+// for every unfinished row of the lane (U independent LDS chains per lane);
+// a round is branch-free (selects, no per-row branches). The S keys of the
+// next iteration are loaded before the current one is probed, so one HBM round
+// trip overlaps each iteration's probing (where registers allow, see PF).
 
 template <int N> __device__ __forceinline__ uint32_t res_get(const uint32_t (&r)[N], int k) {
     return (r[k >> 1] >> ((k & 1) * 16)) & 0xFFFFu;
@@ -691,10 +688,6 @@ template <int N> __device__ __forceinline__ void res_set(uint32_t (&r)[N], int k
 }
 constexpr int kProbeIters = kProbeChunk / (kProbeThreads * kProbeUnroll);
 static_assert(kProbeIters * kProbeThreads * kProbeUnroll == kProbeChunk, "probe chunk shape");
-
-// Rows per lane per iteration of the duplicate-chunk walks (COUNT, EMIT):
-// fewer than kProbeUnroll keep their registers within the kernel's budget.
-constexpr int kDupUnroll = 2;
 
 // First S row of this lane in iteration `it` of a walk with U rows per lane.
 template <int U> __device__ __forceinline__ uint32_t walk_row(const WalkArgs &wa, int it) {
@@ -714,48 +707,46 @@ __device__ __forceinline__ void walk_load(const WalkArgs &wa, int it, Key<KW, NC
         if (row + u * 64 < wa.s_end) k[u] = load_key<KW, NC>(wa.s, row + u * 64);
 }
 
-template <int MODE, int HP, int KW, int NC>
+template <int N> __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+template <int HP, int KW, int NC>
 __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg &hc, uint32_t tag_shift,
-                                               const typename KeyT<KW>::T *rk,
-                                               const uint32_t *slots, uint32_t (&res)[kProbeIters * kProbeUnroll / 2],
-                                               Emitter &em) {
+                                               const typename KeyT<KW>::T *rk, const uint32_t *slots,
+                                               const uint16_t *nxt,
+                                               uint32_t (&res)[kProbeIters * kProbeUnroll / 2]) {
     constexpr int CAP = build_cap(KW * NC);
-    constexpr int U = MODE == WALK_RECORD ? kProbeUnroll : kDupUnroll;
-    constexpr int ITERS = kProbeChunk / (kProbeThreads * U);
-    static_assert(ITERS * kProbeThreads * U == kProbeChunk, "walk shape");
+    constexpr int U = kProbeUnroll;
     // Next-iteration loads double the key registers; they are used where the
     // kernel stays within its register budget (single 8-byte key, FNV / XOR).
-    constexpr bool PF = MODE != WALK_EMIT && KW * NC <= 8 && HP != HP_GENERIC;
+    constexpr bool PF = KW * NC <= 8 && HP != HP_GENERIC;
     const uint32_t bmask = wa.bmask, shift = wa.shift, s_end = wa.s_end;
-    uint32_t total = 0;
+    uint32_t total = 0;  // this lane's matches
     Key<KW, NC> nk[U];
     walk_load<U, KW, NC>(wa, 0, nk);
 #pragma unroll
-    for (int it = 0; it < ITERS; it++) {
+    for (int it = 0; it < kProbeIters; it++) {
         if (walk_wave_row<U>(wa, it) >= s_end) break;  // wave-uniform
         const uint32_t row = walk_row<U>(wa, it);
         Key<KW, NC> sk[U];
-        uint32_t sid[U], st[U];  // st: 16-bit tag << 16 | bucket (buckets < 2^16)
-        uint32_t act = 0;        // bit u: row u of this lane still probing
+        uint32_t st[U];   // 16-bit tag << 16 | bucket (buckets < 2^16)
+        uint32_t act = 0;  // bit u: row u of this lane still probing
         if (!PF && it > 0) walk_load<U, KW, NC>(wa, it, nk);
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const uint32_t i = row + u * 64;
             sk[u] = nk[u];
-            sid[u] = 0;
-            if (MODE == WALK_RECORD) res_set(res, it * U + u, kNoMatch);
-            if (MODE == WALK_EMIT && i < s_end) sid[u] = wa.s.rowids ? wa.s.rowids[i] : i;
-            act |= (i < s_end ? 1u : 0u) << u;
+            res_set(res, it * U + u, kNoMatch);
+            act |= (row + u * 64 < s_end ? 1u : 0u) << u;
         }
-        if (PF && it + 1 < ITERS) walk_load<U, KW, NC>(wa, it + 1, nk);
+        if (PF && it + 1 < kProbeIters) walk_load<U, KW, NC>(wa, it + 1, nk);
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const uint64_t h = hash_key<HP, KW, NC>(sk[u], hc);
             st[u] = (((uint32_t)(h >> tag_shift) & 0xFFFFu) << 16) | ((uint32_t)(h >> shift) & bmask);
         }
-        // A round reads one whole bucket (kBucketSlots entries, one 16-byte LDS
-        // load) for every unfinished row; at load <= 1/2 a row nearly always
-        // ends in its first bucket.
         while (__ballot(act != 0)) {
             uint4 E[U];
 #pragma unroll
@@ -766,14 +757,15 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
                 const uint32_t tg = st[u] >> 16;
                 const uint32_t e4[kBucketSlots] = {E[u].x, E[u].y, E[u].z, E[u].w};
                 uint32_t cand = 0;  // entries whose tag matches, in slot order
-                bool emp = false;   // the bucket has a free slot: the row's keys end here
+                bool emp = false;   // the bucket has a free slot: the row's key cannot be further on
 #pragma unroll
                 for (int j = 0; j < kBucketSlots; j++) {
                     cand |= (e4[j] != 0 && (e4[j] >> 16) == tg ? 1u : 0u) << j;
                     emp |= e4[j] == 0;
                 }
                 if (!live) cand = 0;
-                bool done = false;  // matched, no duplicates: the row is finished
+                bool found = false;
+                uint32_t fo = 0;
                 while (__ballot(cand != 0)) {  // nearly always one pass
                     const bool has = cand != 0;
                     const uint32_t j = __builtin_ctz(cand | (1u << kBucketSlots));
@@ -783,33 +775,37 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
                     bool eq = has;
 #pragma unroll
                     for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == sk[u].c[c];
-                    if (MODE == WALK_EMIT) {
-                        em.put(eq, build_id(wa, o) + wa.ob1, sid[u] + wa.ob1, wa);
-                    } else {
-                        if (MODE == WALK_RECORD) {
-                            const uint32_t cur = res_get(res, it * U + u);
-                            res_set(res, it * U + u, eq ? o : cur);
-                        }
-                        total += __builtin_popcountll(__ballot(eq));
-                    }
-                    if (eq && !wa.dup) {
-                        done = true;
+                    if (eq) {
+                        found = true;
+                        fo = o;
                         cand = 0;
                     }
                 }
-                act &= ~((live && (done || emp) ? 1u : 0u) << u);
+                const uint32_t cur = res_get(res, it * U + u);
+                res_set(res, it * U + u, found ? fo : cur);
+                uint32_t n = found ? 1u : 0u;
+                if (wa.dup) {  // wave-uniform: equal keys of the build chunk are chained behind the entry
+                    uint32_t x = found ? nxt[fo] : 0u;
+                    while (__ballot(x != 0)) {
+                        n += x != 0;
+                        x = x != 0 ? nxt[x - 1] : 0u;
+                    }
+                }
+                total += n;
+                act &= ~((live && (found || emp) ? 1u : 0u) << u);
                 st[u] = (st[u] & 0xFFFF0000u) | ((st[u] + 1) & bmask);
             }
         }
     }
-    return total;
+    return wave_sum<64>(total);
 }
 
-// Writes the matches a WALK_RECORD walk kept. The probe-side and build-side
-// row ids of a group of G iterations are loaded before any is written, so
-// their round trips overlap.
+// Writes the matches probe_walk kept: each matched row's entry, then (when the
+// build chunk has equal keys) the rest of its chain. The probe-side and
+// build-side row ids of a group of G iterations are loaded before any is
+// written, so their round trips overlap.
 template <int G>
-__device__ __forceinline__ void replay_walk(const WalkArgs &wa,
+__device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint16_t *nxt,
                                             const uint32_t (&res)[kProbeIters * kProbeUnroll / 2], Emitter &em) {
     constexpr int U = kProbeUnroll;
     static_assert(kProbeIters % G == 0, "replay groups");
@@ -839,8 +835,18 @@ __device__ __forceinline__ void replay_walk(const WalkArgs &wa,
             if (walk_wave_row<U>(wa, g + it) >= s_end) break;  // wave-uniform
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const bool m = res_get(res, (g + it) * U + u) != kNoMatch;
+                const uint32_t o = res_get(res, (g + it) * U + u);
+                const bool m = o != kNoMatch;
                 em.put(m, bid[it * U + u] + ob1, sid[it * U + u] + ob1, wa);
+                if (wa.dup) {  // wave-uniform
+                    uint32_t x = m ? nxt[o] : 0u;
+                    while (__ballot(x != 0)) {
+                        const bool has = x != 0;
+                        const uint32_t r = has ? x - 1 : 0u;
+                        em.put(has, (has ? build_id(wa, r) : 0u) + ob1, sid[it * U + u] + ob1, wa);
+                        x = has ? nxt[r] : 0u;
+                    }
+                }
             }
         }
     }
@@ -859,6 +865,7 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
     extern __shared__ __align__(16) unsigned char smem[];
     KT *rk = reinterpret_cast<KT *>(smem);                       // [NC][CAP]
     uint32_t *slots = reinterpret_cast<uint32_t *>(rk + NC * CAP);  // [2 * CAP]
+    uint16_t *nxt = reinterpret_cast<uint16_t *>(slots + 2 * CAP);  // [CAP] chain of equal keys (index + 1, 0: end)
     __shared__ ProbeTask t;
     uint32_t res[kProbeIters * kProbeUnroll / 2];  // matched build row per probe row, two per word
 
@@ -937,7 +944,8 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
         if (t.done) break;
         if (w == 0 && lane == 0) next_task = atomicAdd(&a.ctl->task_ticket, 1ull);
 
-        // build: load the R chunk, then insert (index + 1, 16-bit tag) into buckets
+        // build: load the R chunk, then insert (index + 1, 16-bit tag) into buckets;
+        // a key already in the table is chained in front of its entry instead
         const uint32_t r_lo = t.r_lo, r_n = t.r_n, nslots = t.nslots, bmask = (nslots >> 2) - 1, shift = t.shift;
         for (uint32_t i = tid; i < nslots; i += kProbeThreads) slots[i] = 0;
         for (uint32_t i = tid; i < r_n; i += kProbeThreads) {
@@ -954,33 +962,39 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
             const uint32_t tag = (uint32_t)(h >> tag_shift) & 0xFFFFu;
             const uint32_t e = (tag << 16) | (i + 1);
             uint32_t b = (uint32_t)(h >> shift) & bmask;
-            // first free slot of the bucket, else the next bucket (linear probing
-            // over buckets); an equal key met on the way marks the chunk dup
-            while (true) {
-                uint32_t j = 0;
-                for (; j < (uint32_t)kBucketSlots; j++) {
-                    const uint32_t old = atomicCAS(&slots[b * kBucketSlots + j], 0u, e);
-                    if (old == 0) break;
-                    if ((old >> 16) == tag) {
+            // Scan slots in order (the bucket, then the next buckets). An empty
+            // slot takes the row as a new entry; the entry of an equal key takes
+            // it as its new chain head (nxt[i] = old head). nxt[i] is written
+            // before the CAS that publishes the row, and never changes after.
+            bool placed = false;
+            while (!placed) {
+                for (uint32_t j = 0; j < (uint32_t)kBucketSlots && !placed; j++) {
+                    uint32_t *sl = &slots[b * kBucketSlots + j];
+                    nxt[i] = 0;
+                    uint32_t old = atomicCAS(sl, 0u, e);
+                    placed = old == 0;
+                    while (!placed && (old >> 16) == tag) {
                         const uint32_t o = (old & 0xFFFFu) - 1;
                         bool eq = true;
 #pragma unroll
                         for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == k.c[c];
-                        if (eq) t.dup = 1;
+                        if (!eq) break;  // another key with the same tag: go on scanning
+                        t.dup = 1;
+                        nxt[i] = (uint16_t)(o + 1);
+                        const uint32_t prev = atomicCAS(sl, old, e);
+                        placed = prev == old;
+                        old = prev;  // another row of this key became the head first: chain behind it
                     }
                 }
-                if (j < (uint32_t)kBucketSlots) break;
                 b = (b + 1) & bmask;
             }
         }
         __syncthreads();
 
-        // count (recording matches when the chunk has no duplicate keys), reserve
-        // the task's output rows with one atomic, then write them
+        // find and count every row's matches (kept in registers), reserve the
+        // task's output rows with one atomic, then write them
         const WalkArgs wa = walk_args(t);
-        const bool dup = wa.dup;
-        const uint32_t wc = dup ? probe_walk<WALK_COUNT, HP, KW, NC>(wa, hc, tag_shift, rk, slots, res, em)
-                                : probe_walk<WALK_RECORD, HP, KW, NC>(wa, hc, tag_shift, rk, slots, res, em);
+        const uint32_t wc = probe_walk<HP, KW, NC>(wa, hc, tag_shift, rk, slots, nxt, res);
         if (lane == 0) t.wave_count[w] = wc;
         __syncthreads();
         if (tid == 0) {
@@ -992,12 +1006,8 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
         if (wc) {
             em.pos = t.out_base;
             for (uint32_t k = 0; k < w; k++) em.pos += t.wave_count[k];
-            if (dup) {
-                probe_walk<WALK_EMIT, HP, KW, NC>(wa, hc, tag_shift, rk, slots, res, em);
-            } else {
-                // row-id groups of two iterations where registers allow (single 8-byte key, FNV / XOR)
-                replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 2 : 1>(wa, res, em);
-            }
+            // row-id groups of two iterations where registers allow (single 8-byte key, FNV / XOR)
+            replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 2 : 1>(wa, nxt, res, em);
         }
         __syncthreads();
     }
@@ -1035,7 +1045,7 @@ size_t pass_lds_bytes(int kw, int nc) {
 
 size_t probe_lds_bytes(int kw, int nc) {
     const int kb = kw * nc, cap = build_cap(kb);
-    return (size_t)cap * kb + (size_t)cap * 8;
+    return (size_t)cap * kb + (size_t)cap * 8 + (size_t)cap * 2;
 }
 
 }  // namespace

@@ -537,7 +537,6 @@ Err upload(vj_ctx *c, const Plan &pl, hipStream_t s) {
 Err launch_stages(vj_ctx *c, const Plan &pl, const PipeArgs &a, hipStream_t s, bool with_probe, bool events = true) {
     const bool prof = c->profiling && events;
     if (events) VJ_HIP_TRY(hipEventRecord(c->ev[EV_START], s));
-    VJ_LAUNCH(launch_prepare(a, s));
     if (pl.any_part) VJ_LAUNCH(launch_hist(a, pl.shape, s));
     if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_HIST], s));
     VJ_LAUNCH(launch_scan(a, s));

@@ -69,6 +69,43 @@ __device__ __forceinline__ Key<KW, NC> load_key(const TView &t, uint64_t i) {
     return k;
 }
 
+// Pointers in the global address space. Memory operations through them
+// compile to global_* instructions, which count only on vmcnt; through
+// generic pointers they compile to flat_* instructions, which also count on
+// lgkmcnt, so every wait for an LDS result would also wait for the loads in
+// flight (and the compiler must assume a flat store may alias LDS).
+template <typename T> using gptr = __attribute__((address_space(1))) T *;
+template <typename T> __device__ __forceinline__ gptr<T> to_global(T *p) { return (gptr<T>)p; }
+
+// A table's key columns as global pointers (copied out of a TView once, so a
+// loop does not re-read them from its descriptor).
+template <int KW, int NC> struct GKeys {
+    gptr<const typename KeyT<KW>::T> c[NC];
+};
+
+// A TView with global pointers.
+struct GView {
+    gptr<const void> cols[4];
+    gptr<const uint32_t> rowids;  // null: row id = index
+};
+
+template <int KW, int NC>
+__device__ __forceinline__ Key<KW, NC> load_key(const GView &t, uint64_t i) {
+    using T = typename KeyT<KW>::T;
+    Key<KW, NC> k;
+#pragma unroll
+    for (int c = 0; c < NC; c++) k.c[c] = static_cast<gptr<const T>>(t.cols[c])[i];
+    return k;
+}
+
+template <int KW, int NC>
+__device__ __forceinline__ Key<KW, NC> load_key(const GKeys<KW, NC> &g, uint64_t i) {
+    Key<KW, NC> k;
+#pragma unroll
+    for (int c = 0; c < NC; c++) k.c[c] = g.c[c][i];
+    return k;
+}
+
 template <int KW, int NC>
 __device__ __forceinline__ bool key_eq(const Key<KW, NC> &a, const Key<KW, NC> &b) {
     bool eq = true;

@@ -1,13 +1,15 @@
 // The join pipeline's kernels (gfx950). One pipeline runs any number of joins;
 // every kernel draws its tiles from all of them.
 //
-//   k_prepare   reset control words and per-join counters
 //   k_hist      per segment of a job: per-tile pass-1 digit prefixes, the
 //               segment's pass-1 and (pass-1, pass-2) digit counts (LDS)
 //   k_colscan   exclusive scans down the segment columns of those counts
-//   k_bases     per job: partition bases from the column totals
-//   k_tasks     per join: build/probe task prefix over partition pairs
-//   k_scan2     task prefix across joins                             (1 block)
+//   k_bases     per job: partition bases from the column totals; block 0
+//               also resets the task queue and the output counters
+//   k_tasks     per join: build/probe task prefix over partition pairs (one
+//               join: also the queue totals and the task map)
+//   k_scan2     task prefix across joins                 (1 block, >1 join)
+//   k_taskmap   task -> (join, partition) for the first tasks  (>1 join)
 //   k_pass      one radix pass over a tile (pass 1) or a (digit, segment)
 //               chunk (pass 2): per-wave LDS digit ranks -> LDS staging
 //               ordered by digit -> contiguous runs per digit written at the
@@ -42,6 +44,21 @@ __device__ __forceinline__ uint64_t uniform(uint64_t v) {
 }
 template <typename P> __device__ __forceinline__ P *uniform(P *p) {
     return reinterpret_cast<P *>(uniform((uint64_t)reinterpret_cast<uintptr_t>(p)));
+}
+// A wave-uniform generic pointer as a global one in scalar registers.
+template <typename P> __device__ __forceinline__ gptr<P> uniform_global(P *p) { return to_global(uniform(p)); }
+__device__ __forceinline__ GView uniform_view(const TView &t) {
+    GView g;
+#pragma unroll
+    for (int c = 0; c < 4; c++) g.cols[c] = uniform_global(t.cols[c]);
+    g.rowids = uniform_global(t.rowids);
+    return g;
+}
+template <int KW, int NC> __device__ __forceinline__ GKeys<KW, NC> uniform_keys(const TView &t) {
+    GKeys<KW, NC> g;
+#pragma unroll
+    for (int c = 0; c < NC; c++) g.c[c] = uniform_global(static_cast<const typename KeyT<KW>::T *>(t.cols[c]));
+    return g;
 }
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
@@ -160,17 +177,6 @@ __device__ void load_runs(const uint32_t *in, uint32_t n, uint32_t per, uint32_t
 
 // ---------------------------------------------------------------------------
 
-__global__ void k_prepare(PipeArgs a) {
-    if (threadIdx.x == 0) {
-        a.ctl->task_ticket = 0;
-        a.ctl->task_total = 0;
-    }
-    for (uint32_t j = threadIdx.x; j < a.n_joins; j += blockDim.x) {
-        a.out_count[j] = 0;
-        a.join_tasks[j] = 0;
-    }
-}
-
 // One block per segment. The block sweeps the segment one pass-1 tile at a
 // time (all 1024 threads on one tile, coalesced), with one LDS atomic per row
 // into the tile's pass-1 digit counters and, for 2-pass jobs, one into the
@@ -202,7 +208,9 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(PipeArgs a) {
     const HashCfg hc = a.hash;
     const uint32_t t_begin = g * job.seg_tiles;
     const uint32_t t_end = t_begin + job.seg_tiles < job.p1_tiles ? t_begin + job.seg_tiles : job.p1_tiles;
-    uint32_t *tilepre = a.counts + job.tilepre_off;
+    const gptr<uint32_t> tilepre = to_global(a.counts + job.tilepre_off);
+    const GKeys<KW, NC> in = uniform_keys<KW, NC>(job.in);
+    const uint32_t n = job.n;
     for (uint32_t t0 = t_begin; t0 < t_end; t0 += kHistRing) {
         const uint32_t kt = t_end - t0 < (uint32_t)kHistRing ? t_end - t0 : (uint32_t)kHistRing;
         for (uint32_t k = 0; k < kt; k++) {
@@ -210,14 +218,14 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(PipeArgs a) {
             const uint32_t row_lo = (t0 + k) * T + tid;
             Key<KW, NC> key[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
+            for (int u = 0; u < U; u++) {  // all loads in flight at once (past the end: row 0 again)
                 const uint32_t i = row_lo + u * kHistThreads;
-                if (i < job.n) key[u] = load_key<KW, NC>(job.in, i);
+                key[u] = load_key<KW, NC>(in, i < n ? i : 0u);
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const uint32_t i = row_lo + u * kHistThreads;
-                if (i < job.n) {
+                if (i < n) {
                     const uint64_t h = hash_key<HP, KW, NC>(key[u], hc);
                     const uint32_t d1 = (uint32_t)(h >> s1) & m1;
                     atomicAdd(&thk[d1], 1u);
@@ -237,10 +245,10 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(PipeArgs a) {
         }
         __syncthreads();
     }
-    uint32_t *m1row = a.counts + job.m1_off + (uint64_t)g * n1;
+    const gptr<uint32_t> m1row = to_global(a.counts + job.m1_off + (uint64_t)g * n1);
     for (uint32_t d = tid; d < n1; d += kHistThreads) m1row[d] = run[d];
     if (two) {
-        uint32_t *m2row = a.counts + job.m2_off + (uint64_t)g * nb;
+        const gptr<uint32_t> m2row = to_global(a.counts + job.m2_off + (uint64_t)g * nb);
         for (uint32_t q = tid; q < nb; q += kHistThreads) m2row[q] = h2[q];
     }
 }
@@ -291,9 +299,16 @@ __global__ __launch_bounds__(1024) void k_colscan(PipeArgs a) {
 
 // One block of 1024 threads per job: partition bases = exclusive scan of the
 // column totals (the last row of m2, or of m1 for one pass).
+// Block 0 also clears the task queue ticket and the output counters for the
+// build+probe kernel, which runs after this one.
 __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
     __shared__ uint32_t sh32[20];
     extern __shared__ uint32_t lds_runs[];  // nb + nb/32 words
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) a.ctl->task_ticket = 0;
+        for (uint32_t j = threadIdx.x; j < a.n_joins; j += blockDim.x) a.out_count[j] = 0;
+    }
+    if (blockIdx.x >= a.n_jobs) return;
     const JobDesc &job = a.jobs[blockIdx.x];
     uint32_t *base = a.base + job.base_off;
     if (job.b1 == 0) {
@@ -365,7 +380,8 @@ __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
         sum += t[k];
     }
     unsigned long long total;
-    unsigned long long pre = block_excl_scan<unsigned long long>(sum, sh64, &total);
+    const unsigned long long pre0 = block_excl_scan<unsigned long long>(sum, sh64, &total);
+    unsigned long long pre = pre0;
 #pragma unroll
     for (int k = 0; k < kScanChunk; k++) {
         if ((uint32_t)k < per && lo + k < n) pt[lo + k] = pre;
@@ -374,6 +390,20 @@ __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
     if (threadIdx.x == 0) {
         pt[n] = total;
         a.join_tasks[jn] = total;
+    }
+    if (a.n_joins != 1) return;
+    // One join: its task prefix is the whole queue, so the queue totals and
+    // the task -> partition map are written here.
+    if (threadIdx.x == 0) {
+        a.task0[0] = 0;
+        a.task0[1] = total;
+        a.ctl->task_total = total;
+    }
+    pre = pre0;
+    for (int k = 0; k < kScanChunk; k++) {
+        if ((uint32_t)k >= per || lo + k >= n) break;
+        for (unsigned long long q = pre; q < pre + t[k] && q < a.taskmap_cap; q++) a.taskmap[q] = lo + k;
+        pre += t[k];
     }
 }
 
@@ -504,11 +534,12 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>()))
     }
 
     const uint32_t bins = ti.bins, shift = ti.shift, mask = bins - 1;
-    const TView src = ti.src;
-    KT *dcols[NC];  // destination columns, in scalar registers
+    const GKeys<KW, NC> skeys_g = uniform_keys<KW, NC>(ti.src);  // source columns, in scalar registers
+    const gptr<const uint32_t> sids_g = uniform_global(ti.src.rowids);
+    gptr<KT> dcols[NC];  // destination columns
 #pragma unroll
-    for (int c = 0; c < NC; c++) dcols[c] = static_cast<KT *>(uniform(ti.dst_cols[c]));
-    uint32_t *const dids = uniform(ti.dst_ids);
+    for (int c = 0; c < NC; c++) dcols[c] = uniform_global(static_cast<KT *>(ti.dst_cols[c]));
+    const gptr<uint32_t> dids = uniform_global(ti.dst_ids);
     const HashCfg hc = a.hash;
     for (uint32_t lo = ti.lo; lo < ti.hi; lo += T) {
         const uint32_t hi = lo + T < ti.hi ? lo + T : ti.hi;
@@ -520,13 +551,21 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>()))
         uint32_t ids[ITEMS];
         uint32_t pk[ITEMS];  // digit | rank << 8, or ~0 for no row
         const uint32_t wbase = lo + w * 64 * ITEMS + lane;
+        // every load issued before the first use (past the end: row lo again)
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
             const uint32_t i = wbase + it * 64;
-            if (i < hi) {
-                keys[it] = load_key<KW, NC>(src, i);
-                ids[it] = src.rowids ? src.rowids[i] : i;
+            keys[it] = load_key<KW, NC>(skeys_g, i < hi ? i : lo);
+        }
+        if (sids_g) {  // wave-uniform
+#pragma unroll
+            for (int it = 0; it < ITEMS; it++) {
+                const uint32_t i = wbase + it * 64;
+                ids[it] = sids_g[i < hi ? i : lo];
             }
+        } else {
+#pragma unroll
+            for (int it = 0; it < ITEMS; it++) ids[it] = wbase + it * 64;
         }
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
@@ -614,10 +653,10 @@ constexpr uint16_t kNoMatch = 0xFFFF;
 
 // One task's parameters for the walks (all wave-uniform).
 struct WalkArgs {
-    TView s;
-    const uint32_t *r_ids;  // build rows' ids (null: the row index)
+    GView s;
+    gptr<const uint32_t> r_ids;  // build rows' ids (null: the row index)
     uint32_t r_lo;
-    uint32_t *out_r, *out_s;
+    gptr<uint32_t> out_r, out_s;
     uint64_t cap;
     uint32_t s_lo, s_end, bmask, shift, ob1;
     bool dup;
@@ -625,13 +664,11 @@ struct WalkArgs {
 
 __device__ __forceinline__ WalkArgs walk_args(const ProbeTask &t) {
     WalkArgs w;
-    for (int c = 0; c < 4; c++) w.s.cols[c] = uniform(t.s.cols[c]);
-    w.s.rowids = uniform(t.s.rowids);
-    w.s.n = 0;
-    w.r_ids = uniform(t.r.rowids);
+    w.s = uniform_view(t.s);
+    w.r_ids = uniform_global(t.r.rowids);
     w.r_lo = uniform(t.r_lo);
-    w.out_r = uniform(t.out_r);
-    w.out_s = uniform(t.out_s);
+    w.out_r = uniform_global(t.out_r);
+    w.out_s = uniform_global(t.out_s);
     w.cap = uniform((uint64_t)t.cap);
     w.s_lo = uniform(t.s_lo);
     w.s_end = uniform(t.s_lo + t.s_n);
@@ -948,8 +985,9 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
         // a key already in the table is chained in front of its entry instead
         const uint32_t r_lo = t.r_lo, r_n = t.r_n, nslots = t.nslots, bmask = (nslots >> 2) - 1, shift = t.shift;
         for (uint32_t i = tid; i < nslots; i += kProbeThreads) slots[i] = 0;
+        const GView rv = uniform_view(t.r);
         for (uint32_t i = tid; i < r_n; i += kProbeThreads) {
-            Key<KW, NC> k = load_key<KW, NC>(t.r, r_lo + i);
+            Key<KW, NC> k = load_key<KW, NC>(rv, r_lo + i);
 #pragma unroll
             for (int c = 0; c < NC; c++) rk[c * CAP + i] = k.c[c];
         }
@@ -1050,11 +1088,6 @@ size_t probe_lds_bytes(int kw, int nc) {
 
 }  // namespace
 
-hipError_t launch_prepare(const PipeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-
 hipError_t launch_hist(const PipeArgs &a, const LaunchShape &sh, hipStream_t s) {
     if (a.total_hist_blocks == 0) return hipSuccess;
     const size_t lds = (size_t)a.max_hist_bins * 4;
@@ -1071,12 +1104,13 @@ hipError_t launch_scan(const PipeArgs &a, hipStream_t s) {
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     const size_t runs_lds = (size_t)(lds_pad((1u << kMaxHistBits) + 1) + 1) * 4;  // load_runs of <= 2^14 + 1 words
-    hipLaunchKernelGGL(k_bases, dim3(a.n_jobs), dim3(1024), runs_lds, s, a);
+    hipLaunchKernelGGL(k_bases, dim3(a.n_jobs ? a.n_jobs : 1), dim3(1024), runs_lds, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.n_joins) {
         hipLaunchKernelGGL(k_tasks, dim3(a.n_joins), dim3(1024), runs_lds, s, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
+    if (a.n_joins == 1) return hipSuccess;  // k_tasks wrote the queue totals and the task map
     hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.n_joins && a.taskmap_cap) {

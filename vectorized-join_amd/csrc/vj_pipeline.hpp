@@ -91,7 +91,8 @@ struct ColScanDesc {
     uint32_t pad;
 };
 
-// Control words, reset by k_prepare at the start of every pipeline.
+// Control words: the ticket is reset by k_bases, the total written by
+// k_tasks (one join) or k_scan2, before the build+probe kernel reads them.
 struct Ctl {
     unsigned long long task_ticket;
     unsigned long long task_total;
@@ -126,7 +127,6 @@ struct LaunchShape {
     int hp, kw, nc;  // hash path, key width, key columns
     int cus;         // compute units of the device
 };
-hipError_t launch_prepare(const PipeArgs &a, hipStream_t s);
 hipError_t launch_hist(const PipeArgs &a, const LaunchShape &sh, hipStream_t s);
 hipError_t launch_scan(const PipeArgs &a, hipStream_t s);
 hipError_t launch_reset_probe(const PipeArgs &a, hipStream_t s);

@@ -130,7 +130,9 @@ template <typename T> __device__ T block_excl_scan(T v, T *sh, T *total) {
 // out. Each thread takes a contiguous run of elements, read in register
 // chunks of kScanChunk so the loads of a chunk are in flight together.
 constexpr int kScanChunk = 16;
-template <typename T> __device__ void block_scan_array(const T *in, T *out, uint32_t n, T *sh) {
+template <typename T> __device__ void block_scan_array(const T *in_, T *out_, uint32_t n, T *sh) {
+    const gptr<const T> in = to_global(in_);
+    const gptr<T> out = to_global(out_);
     const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
     const uint32_t lo = threadIdx.x * per < n ? threadIdx.x * per : n;
     const uint32_t hi = lo + per < n ? lo + per : n;
@@ -166,7 +168,19 @@ template <typename T> __device__ void block_scan_array(const T *in, T *out, uint
 __host__ __device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 5); }
 __device__ void load_runs(const uint32_t *in, uint32_t n, uint32_t per, uint32_t (&v)[kScanChunk], uint32_t *next,
                           uint32_t *lds) {
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[lds_pad(i)] = in[i];
+    // every thread's loads are issued before any is stored (n <= (kScanChunk + 1) * blockDim.x, n >= 1)
+    const gptr<const uint32_t> g = to_global(in);
+    uint32_t x[kScanChunk + 1];
+#pragma unroll
+    for (int r = 0; r <= kScanChunk; r++) {
+        const uint32_t i = threadIdx.x + r * blockDim.x;
+        x[r] = g[i < n ? i : n - 1];
+    }
+#pragma unroll
+    for (int r = 0; r <= kScanChunk; r++) {
+        const uint32_t i = threadIdx.x + r * blockDim.x;
+        if (i < n) lds[lds_pad(i)] = x[r];
+    }
     __syncthreads();
     const uint32_t lo = threadIdx.x * per;
 #pragma unroll
@@ -310,7 +324,8 @@ __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
     }
     if (blockIdx.x >= a.n_jobs) return;
     const JobDesc &job = a.jobs[blockIdx.x];
-    uint32_t *base = a.base + job.base_off;
+    const gptr<uint32_t> base = to_global(a.base + job.base_off);
+    const gptr<uint32_t> keep = to_global(job.keep_base);  // null: not kept
     if (job.b1 == 0) {
         if (threadIdx.x == 0) {
             base[0] = 0;
@@ -320,11 +335,25 @@ __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
     }
     const uint32_t nb = 1u << (job.b1 + job.b2);
     if (job.res_base) {  // resident table: its bases were computed when it was built
-        for (uint32_t q = threadIdx.x; q <= nb; q += blockDim.x) base[q] = job.res_base[q];
+        const gptr<const uint32_t> rb = to_global(job.res_base);
+        uint32_t x[kScanChunk + 1];  // nb + 1 <= (kScanChunk + 1) * blockDim.x; loads first, then stores
+#pragma unroll
+        for (int r = 0; r <= kScanChunk; r++) {
+            const uint32_t q = threadIdx.x + r * blockDim.x;
+            x[r] = rb[q <= nb ? q : nb];
+        }
+#pragma unroll
+        for (int r = 0; r <= kScanChunk; r++) {
+            const uint32_t q = threadIdx.x + r * blockDim.x;
+            if (q <= nb) base[q] = x[r];
+        }
         return;
     }
     if (job.n == 0) {  // an empty job has no count rows
-        for (uint32_t q = threadIdx.x; q <= nb; q += blockDim.x) base[q] = 0;
+        for (uint32_t q = threadIdx.x; q <= nb; q += blockDim.x) {
+            base[q] = 0;
+            if (keep) keep[q] = 0;
+        }
     } else {
         const uint32_t *tot = a.counts + (job.b2 ? job.m2_off : job.m1_off) + (uint64_t)job.nseg * nb;
         const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;  // <= kScanChunk (nb <= 2^14)
@@ -338,15 +367,16 @@ __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
         const uint32_t lo = threadIdx.x * per;
 #pragma unroll
         for (int j = 0; j < kScanChunk; j++) {
-            if ((uint32_t)j < per && lo + j < nb) base[lo + j] = pre;
+            if ((uint32_t)j < per && lo + j < nb) {
+                base[lo + j] = pre;
+                if (keep) keep[lo + j] = pre;
+            }
             pre += v[j];
         }
-        if (threadIdx.x == 0) base[nb] = total;
-        __syncthreads();
-    }
-    if (job.keep_base) {
-        __syncthreads();
-        for (uint32_t q = threadIdx.x; q <= nb; q += blockDim.x) job.keep_base[q] = base[q];
+        if (threadIdx.x == 0) {
+            base[nb] = total;
+            if (keep) keep[nb] = total;
+        }
     }
 }
 
@@ -364,7 +394,7 @@ __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
     const JoinDesc &jd = a.joins[jn];
     const uint32_t *rb = a.base + a.jobs[jd.job_r].base_off;
     const uint32_t *sb = a.base + a.jobs[jd.job_s].base_off;
-    unsigned long long *pt = a.ptask + jd.ptask_off;
+    const gptr<unsigned long long> pt = to_global(a.ptask + jd.ptask_off);
     const uint32_t n = jd.nparts, per = (n + blockDim.x - 1) / blockDim.x;  // <= kScanChunk (nparts <= 2^14)
     uint32_t r[kScanChunk], q[kScanChunk], rn, qn;
     load_runs(rb, n + 1, per, r, &rn, lds_runs);
@@ -400,9 +430,10 @@ __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
         a.ctl->task_total = total;
     }
     pre = pre0;
+    const gptr<unsigned long long> tmap = to_global(a.taskmap);
     for (int k = 0; k < kScanChunk; k++) {
         if ((uint32_t)k >= per || lo + k >= n) break;
-        for (unsigned long long q = pre; q < pre + t[k] && q < a.taskmap_cap; q++) a.taskmap[q] = lo + k;
+        for (unsigned long long q = pre; q < pre + t[k] && q < a.taskmap_cap; q++) tmap[q] = lo + k;
         pre += t[k];
     }
 }

@@ -1016,11 +1016,26 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
         // a key already in the table is chained in front of its entry instead
         const uint32_t r_lo = t.r_lo, r_n = t.r_n, nslots = t.nslots, bmask = (nslots >> 2) - 1, shift = t.shift;
         for (uint32_t i = tid; i < nslots; i += kProbeThreads) slots[i] = 0;
-        const GView rv = uniform_view(t.r);
-        for (uint32_t i = tid; i < r_n; i += kProbeThreads) {
-            Key<KW, NC> k = load_key<KW, NC>(rv, r_lo + i);
+        {
+            // two R rows per lane in flight at once (a task has r_n >= 1)
+            constexpr int RB = 2;
+            const GView rv = uniform_view(t.r);
+            for (uint32_t i0 = 0; i0 < r_n; i0 += RB * kProbeThreads) {
+                Key<KW, NC> rkey[RB];
 #pragma unroll
-            for (int c = 0; c < NC; c++) rk[c * CAP + i] = k.c[c];
+                for (int r = 0; r < RB; r++) {
+                    const uint32_t i = i0 + tid + r * kProbeThreads;
+                    rkey[r] = load_key<KW, NC>(rv, r_lo + (i < r_n ? i : 0u));
+                }
+#pragma unroll
+                for (int r = 0; r < RB; r++) {
+                    const uint32_t i = i0 + tid + r * kProbeThreads;
+                    if (i < r_n) {
+#pragma unroll
+                        for (int c = 0; c < NC; c++) rk[c * CAP + i] = rkey[r].c[c];
+                    }
+                }
+            }
         }
         __syncthreads();
         for (uint32_t i = tid; i < r_n; i += kProbeThreads) {

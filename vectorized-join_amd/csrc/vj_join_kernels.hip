@@ -126,6 +126,48 @@ template <typename T> __device__ T block_excl_scan(T v, T *sh, T *total) {
     return r;
 }
 
+// Exclusive scan of n <= 64 * kScanRounds * (blockDim.x / 64) values by one
+// block. Wave w owns a contiguous span of the elements; in round j lane l
+// holds element span * w + 64 * j + l, so every load and store of a round is
+// one contiguous run. get(e, ok) returns element e (0 where !ok; it must not
+// read memory past the end for !ok), put(e, prefix, value) consumes the result.
+// Returns the total; block-wide.
+constexpr int kScanRounds = 16;
+template <typename T, typename Get, typename Put>
+__device__ T block_scan_spans(uint32_t n, T *sh, Get get, Put put) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t span = ((n + nw - 1) / nw + 63) & ~63u;
+    const uint32_t e0 = w * span + lane;
+    T v[kScanRounds];
+    T sum = 0;
+#pragma unroll
+    for (int j = 0; j < kScanRounds; j++) {
+        const uint32_t e = e0 + j * 64;
+        v[j] = get(e, (uint32_t)j * 64 < span && e < n);
+        sum += v[j];
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
+    if (lane == 0) sh[w] = sum;
+    __syncthreads();
+    T carry = 0, total = 0;
+    for (uint32_t k = 0; k < nw; k++) {
+        const T x = sh[k];
+        carry += k < w ? x : T(0);
+        total += x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kScanRounds; j++) {
+        if ((uint32_t)j * 64 >= span) break;  // wave-uniform
+        const uint32_t e = e0 + j * 64;
+        const T inc = wave_incl_scan(v[j]);
+        if (e < n) put(e, carry + inc - v[j], v[j]);
+        carry += __shfl(inc, 63, 64);
+    }
+    return total;
+}
+
 // Exclusive scan of in[0..n) into out[0..n], total into out[n]; in may equal
 // out. Each thread takes a contiguous run of elements, read in register
 // chunks of kScanChunk so the loads of a chunk are in flight together.
@@ -157,35 +199,6 @@ template <typename T> __device__ void block_scan_array(const T *in_, T *out_, ui
         }
     }
     if (threadIdx.x == 0) out[n] = total;
-    __syncthreads();
-}
-
-// Loads in[0..n) so that thread t holds its contiguous run
-// [t*per, t*per + per) in v[0..per) (per <= kScanChunk): coalesced global loads
-// into LDS (`lds` holds n + n/32 words, padded against bank conflicts), then
-// each thread reads its run. Also returns in[t*per + per] (the element after
-// the run, if < n) in *next. Block-wide; ends with a barrier.
-__host__ __device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 5); }
-__device__ void load_runs(const uint32_t *in, uint32_t n, uint32_t per, uint32_t (&v)[kScanChunk], uint32_t *next,
-                          uint32_t *lds) {
-    // every thread's loads are issued before any is stored (n <= (kScanChunk + 1) * blockDim.x, n >= 1)
-    const gptr<const uint32_t> g = to_global(in);
-    uint32_t x[kScanChunk + 1];
-#pragma unroll
-    for (int r = 0; r <= kScanChunk; r++) {
-        const uint32_t i = threadIdx.x + r * blockDim.x;
-        x[r] = g[i < n ? i : n - 1];
-    }
-#pragma unroll
-    for (int r = 0; r <= kScanChunk; r++) {
-        const uint32_t i = threadIdx.x + r * blockDim.x;
-        if (i < n) lds[lds_pad(i)] = x[r];
-    }
-    __syncthreads();
-    const uint32_t lo = threadIdx.x * per;
-#pragma unroll
-    for (int j = 0; j < kScanChunk; j++) v[j] = (uint32_t)j < per && lo + j < n ? lds[lds_pad(lo + j)] : 0u;
-    *next = lo + per < n ? lds[lds_pad(lo + per)] : 0u;
     __syncthreads();
 }
 
@@ -317,7 +330,6 @@ __global__ __launch_bounds__(1024) void k_colscan(PipeArgs a) {
 // build+probe kernel, which runs after this one.
 __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
     __shared__ uint32_t sh32[20];
-    extern __shared__ uint32_t lds_runs[];  // nb + nb/32 words
     if (blockIdx.x == 0) {
         if (threadIdx.x == 0) a.ctl->task_ticket = 0;
         for (uint32_t j = threadIdx.x; j < a.n_joins; j += blockDim.x) a.out_count[j] = 0;
@@ -355,24 +367,13 @@ __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
             if (keep) keep[q] = 0;
         }
     } else {
-        const uint32_t *tot = a.counts + (job.b2 ? job.m2_off : job.m1_off) + (uint64_t)job.nseg * nb;
-        const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;  // <= kScanChunk (nb <= 2^14)
-        uint32_t v[kScanChunk], nx;
-        load_runs(tot, nb, per, v, &nx, lds_runs);
-        uint32_t sum = 0;
-#pragma unroll
-        for (int j = 0; j < kScanChunk; j++) sum += v[j];
-        uint32_t total;
-        uint32_t pre = block_excl_scan<uint32_t>(sum, sh32, &total);
-        const uint32_t lo = threadIdx.x * per;
-#pragma unroll
-        for (int j = 0; j < kScanChunk; j++) {
-            if ((uint32_t)j < per && lo + j < nb) {
-                base[lo + j] = pre;
-                if (keep) keep[lo + j] = pre;
-            }
-            pre += v[j];
-        }
+        const gptr<const uint32_t> tot = to_global(a.counts + (job.b2 ? job.m2_off : job.m1_off) + (uint64_t)job.nseg * nb);
+        const uint32_t total = block_scan_spans<uint32_t>(  // nb <= 2^14
+            nb, sh32, [&](uint32_t e, bool ok) { const uint32_t x = tot[ok ? e : 0u]; return ok ? x : 0u; },
+            [&](uint32_t e, uint32_t pre, uint32_t) {
+                base[e] = pre;
+                if (keep) keep[e] = pre;
+            });
         if (threadIdx.x == 0) {
             base[nb] = total;
             if (keep) keep[nb] = total;
@@ -381,60 +382,44 @@ __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
 }
 
 // One block of 1024 threads per join: build/probe tasks = R chunks x S chunks
-// of every partition pair, as an exclusive prefix over partitions. Each thread
-// takes a contiguous run of partitions; their bases are read in register chunks.
+// of every partition pair, as an exclusive prefix over partitions.
 __device__ __forceinline__ unsigned long long part_tasks(uint32_t nr, uint32_t ns, const JoinDesc &jd) {
     return nr && ns ? (unsigned long long)((nr + jd.cap_r - 1) / jd.cap_r) * ((ns + jd.cap_s - 1) / jd.cap_s) : 0ull;
 }
 
 __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
     __shared__ unsigned long long sh64[20];
-    extern __shared__ uint32_t lds_runs[];  // (nparts + 1) + (nparts + 1)/32 words
     const uint32_t jn = blockIdx.x;
     const JoinDesc &jd = a.joins[jn];
-    const uint32_t *rb = a.base + a.jobs[jd.job_r].base_off;
-    const uint32_t *sb = a.base + a.jobs[jd.job_s].base_off;
+    const gptr<const uint32_t> rb = to_global(a.base + a.jobs[jd.job_r].base_off);
+    const gptr<const uint32_t> sb = to_global(a.base + a.jobs[jd.job_s].base_off);
     const gptr<unsigned long long> pt = to_global(a.ptask + jd.ptask_off);
-    const uint32_t n = jd.nparts, per = (n + blockDim.x - 1) / blockDim.x;  // <= kScanChunk (nparts <= 2^14)
-    uint32_t r[kScanChunk], q[kScanChunk], rn, qn;
-    load_runs(rb, n + 1, per, r, &rn, lds_runs);
-    load_runs(sb, n + 1, per, q, &qn, lds_runs);
-    const uint32_t lo = threadIdx.x * per;
-    unsigned long long t[kScanChunk];
-    unsigned long long sum = 0;
-#pragma unroll
-    for (int k = 0; k < kScanChunk; k++) {
-        const uint32_t r1 = k + 1 < kScanChunk && (uint32_t)(k + 1) < per ? r[k + 1] : rn;
-        const uint32_t q1 = k + 1 < kScanChunk && (uint32_t)(k + 1) < per ? q[k + 1] : qn;
-        t[k] = (uint32_t)k < per && lo + k < n ? part_tasks(r1 - r[k], q1 - q[k], jd) : 0ull;
-        sum += t[k];
-    }
-    unsigned long long total;
-    const unsigned long long pre0 = block_excl_scan<unsigned long long>(sum, sh64, &total);
-    unsigned long long pre = pre0;
-#pragma unroll
-    for (int k = 0; k < kScanChunk; k++) {
-        if ((uint32_t)k < per && lo + k < n) pt[lo + k] = pre;
-        pre += t[k];
-    }
+    const gptr<unsigned long long> tmap = to_global(a.taskmap);
+    const uint32_t n = jd.nparts;  // <= 2^14; rb, sb hold n + 1 bases
+    // One join: its task prefix is the whole queue, so the task -> partition
+    // map and the queue totals are written here.
+    const bool one = a.n_joins == 1;
+    const unsigned long long map_cap = a.taskmap_cap;
+    const unsigned long long total = block_scan_spans<unsigned long long>(
+        n, sh64,
+        [&](uint32_t e, bool ok) {
+            const uint32_t i = ok ? e : 0u;
+            const uint32_t r0 = rb[i], r1 = rb[i + 1], q0 = sb[i], q1 = sb[i + 1];
+            return ok ? part_tasks(r1 - r0, q1 - q0, jd) : 0ull;
+        },
+        [&](uint32_t e, unsigned long long pre, unsigned long long t) {
+            pt[e] = pre;
+            if (one)
+                for (unsigned long long q = pre; q < pre + t && q < map_cap; q++) tmap[q] = e;
+        });
     if (threadIdx.x == 0) {
         pt[n] = total;
         a.join_tasks[jn] = total;
-    }
-    if (a.n_joins != 1) return;
-    // One join: its task prefix is the whole queue, so the queue totals and
-    // the task -> partition map are written here.
-    if (threadIdx.x == 0) {
-        a.task0[0] = 0;
-        a.task0[1] = total;
-        a.ctl->task_total = total;
-    }
-    pre = pre0;
-    const gptr<unsigned long long> tmap = to_global(a.taskmap);
-    for (int k = 0; k < kScanChunk; k++) {
-        if ((uint32_t)k >= per || lo + k >= n) break;
-        for (unsigned long long q = pre; q < pre + t[k] && q < a.taskmap_cap; q++) tmap[q] = lo + k;
-        pre += t[k];
+        if (one) {
+            a.task0[0] = 0;
+            a.task0[1] = total;
+            a.ctl->task_total = total;
+        }
     }
 }
 
@@ -1149,11 +1134,10 @@ hipError_t launch_scan(const PipeArgs &a, hipStream_t s) {
         hipLaunchKernelGGL(k_colscan, dim3(a.n_colscan), dim3(1024), 0, s, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    const size_t runs_lds = (size_t)(lds_pad((1u << kMaxHistBits) + 1) + 1) * 4;  // load_runs of <= 2^14 + 1 words
-    hipLaunchKernelGGL(k_bases, dim3(a.n_jobs ? a.n_jobs : 1), dim3(1024), runs_lds, s, a);
+    hipLaunchKernelGGL(k_bases, dim3(a.n_jobs ? a.n_jobs : 1), dim3(1024), 0, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.n_joins) {
-        hipLaunchKernelGGL(k_tasks, dim3(a.n_joins), dim3(1024), runs_lds, s, a);
+        hipLaunchKernelGGL(k_tasks, dim3(a.n_joins), dim3(1024), 0, s, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (a.n_joins == 1) return hipSuccess;  // k_tasks wrote the queue totals and the task map

@@ -173,7 +173,6 @@ Err check_pair(const vj_table *r, const vj_table *s) {
     return Err::ok();
 }
 
-constexpr uint32_t kMaxSegments = 512;  // segments (histogram blocks) per job
 
 // Bytes of a resident table of n rows.
 size_t built_bytes(uint64_t n, int nc, int kw, uint32_t b1, uint32_t b2) {

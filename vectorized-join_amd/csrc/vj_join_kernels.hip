@@ -128,24 +128,26 @@ template <typename T> __device__ T block_excl_scan(T v, T *sh, T *total) {
 
 // Exclusive scan of n <= 64 * kScanRounds * (blockDim.x / 64) values by one
 // block. Wave w owns a contiguous span of the elements; in round j lane l
-// holds element span * w + 64 * j + l, so every load and store of a round is
-// one contiguous run. get(e, ok) returns element e (0 where !ok; it must not
-// read memory past the end for !ok), put(e, prefix, value) consumes the result.
-// Returns the total; block-wide.
+// holds element span * w + 64 * j + l (Spans::elem), so every load and store
+// of a round is one contiguous run. The caller fills v[j] (0 where !ok(j));
+// put(e, prefix, value) consumes the results. Returns the total; block-wide.
 constexpr int kScanRounds = 16;
-template <typename T, typename Get, typename Put>
-__device__ T block_scan_spans(uint32_t n, T *sh, Get get, Put put) {
+struct Spans {
+    uint32_t e0, span, n;
+    __device__ explicit Spans(uint32_t n_) : n(n_) {
+        const uint32_t nw = blockDim.x >> 6;
+        span = ((n + nw - 1) / nw + 63) & ~63u;
+        e0 = (threadIdx.x >> 6) * span + (threadIdx.x & 63);
+    }
+    __device__ uint32_t elem(int j) const { return e0 + (uint32_t)j * 64; }
+    __device__ bool ok(int j) const { return (uint32_t)j * 64 < span && elem(j) < n; }
+};
+template <typename T, typename Put>
+__device__ T block_scan_spans(const Spans &sp, const T (&v)[kScanRounds], T *sh, Put put) {
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const uint32_t span = ((n + nw - 1) / nw + 63) & ~63u;
-    const uint32_t e0 = w * span + lane;
-    T v[kScanRounds];
     T sum = 0;
 #pragma unroll
-    for (int j = 0; j < kScanRounds; j++) {
-        const uint32_t e = e0 + j * 64;
-        v[j] = get(e, (uint32_t)j * 64 < span && e < n);
-        sum += v[j];
-    }
+    for (int j = 0; j < kScanRounds; j++) sum += v[j];
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
     if (lane == 0) sh[w] = sum;
@@ -159,10 +161,9 @@ __device__ T block_scan_spans(uint32_t n, T *sh, Get get, Put put) {
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kScanRounds; j++) {
-        if ((uint32_t)j * 64 >= span) break;  // wave-uniform
-        const uint32_t e = e0 + j * 64;
+        if ((uint32_t)j * 64 >= sp.span) break;  // wave-uniform
         const T inc = wave_incl_scan(v[j]);
-        if (e < n) put(e, carry + inc - v[j], v[j]);
+        if (sp.ok(j)) put(sp.elem(j), carry + inc - v[j], v[j]);
         carry += __shfl(inc, 63, 64);
     }
     return total;
@@ -282,43 +283,41 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(PipeArgs a) {
 
 // Exclusive scans down kColScanCols columns of a [nseg + 1][ncols] count
 // matrix (the last row receives the column totals). 1024 threads = 64 columns
-// x 16 row groups: each thread reads its rows in register chunks of 16, the
-// groups' sums are combined in LDS, and the rows are rewritten with their
-// prefixes.
+// x 16 row groups: each thread reads its rows once into registers (all loads
+// in flight together), the groups' sums are combined in LDS, and the rows are
+// rewritten with their prefixes.
 constexpr int kColScanGroups = 1024 / kColScanCols;
+constexpr int kColScanRows = (kMaxSegments + kColScanGroups - 1) / kColScanGroups;
 __global__ __launch_bounds__(1024) void k_colscan(PipeArgs a) {
     __shared__ uint32_t part[kColScanGroups][kColScanCols];
     const ColScanDesc d = a.colscan[blockIdx.x];
     const JobDesc &job = a.jobs[d.job];
     const uint32_t ncols = d.which == 1 ? (1u << job.b1) : (1u << (job.b1 + job.b2));
-    uint32_t *m = a.counts + (d.which == 1 ? job.m1_off : job.m2_off);
+    const gptr<uint32_t> m = to_global(a.counts + (d.which == 1 ? job.m1_off : job.m2_off));
     const uint32_t cc = threadIdx.x % kColScanCols, rg = threadIdx.x / kColScanCols;
     const uint32_t c = d.col0 + cc;
-    const uint32_t rows = job.nseg, per = (rows + kColScanGroups - 1) / kColScanGroups;
+    const uint32_t rows = job.nseg, per = (rows + kColScanGroups - 1) / kColScanGroups;  // <= kColScanRows
     const uint32_t lo = rg * per < rows ? rg * per : rows, hi = lo + per < rows ? lo + per : rows;
     const bool col_ok = c < ncols;
+    const uint32_t cl = col_ok ? c : 0u;
+    uint32_t v[kColScanRows];
     uint32_t sum = 0;
-    for (uint32_t b = lo; b < hi; b += 16) {
-        uint32_t v[16];
 #pragma unroll
-        for (int r = 0; r < 16; r++) v[r] = col_ok && b + r < hi ? m[(uint64_t)(b + r) * ncols + c] : 0u;
-#pragma unroll
-        for (int r = 0; r < 16; r++) sum += v[r];
+    for (int r = 0; r < kColScanRows; r++) {  // rows past hi read row lo (<= rows: in bounds) and count 0
+        const bool ok = col_ok && lo + r < hi;
+        const uint32_t x = m[(uint64_t)(ok ? lo + r : lo) * ncols + cl];
+        v[r] = ok ? x : 0u;
+        sum += v[r];
     }
     part[rg][cc] = sum;
     __syncthreads();
     uint32_t pre = 0;
     for (uint32_t k = 0; k < rg; k++) pre += part[k][cc];
     if (col_ok) {
-        for (uint32_t b = lo; b < hi; b += 16) {
-            uint32_t v[16];
 #pragma unroll
-            for (int r = 0; r < 16; r++) v[r] = b + r < hi ? m[(uint64_t)(b + r) * ncols + c] : 0u;
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                if (b + r < hi) m[(uint64_t)(b + r) * ncols + c] = pre;
-                pre += v[r];
-            }
+        for (int r = 0; r < kColScanRows; r++) {
+            if (lo + r < hi) m[(uint64_t)(lo + r) * ncols + c] = pre;
+            pre += v[r];
         }
         if (rg == kColScanGroups - 1) m[(uint64_t)rows * ncols + c] = pre;
     }
@@ -368,12 +367,17 @@ __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
         }
     } else {
         const gptr<const uint32_t> tot = to_global(a.counts + (job.b2 ? job.m2_off : job.m1_off) + (uint64_t)job.nseg * nb);
-        const uint32_t total = block_scan_spans<uint32_t>(  // nb <= 2^14
-            nb, sh32, [&](uint32_t e, bool ok) { const uint32_t x = tot[ok ? e : 0u]; return ok ? x : 0u; },
-            [&](uint32_t e, uint32_t pre, uint32_t) {
-                base[e] = pre;
-                if (keep) keep[e] = pre;
-            });
+        const Spans sp(nb);  // nb <= 2^14
+        uint32_t v[kScanRounds];
+#pragma unroll
+        for (int j = 0; j < kScanRounds; j++) {
+            const uint32_t x = tot[sp.ok(j) ? sp.elem(j) : 0u];
+            v[j] = sp.ok(j) ? x : 0u;
+        }
+        const uint32_t total = block_scan_spans<uint32_t>(sp, v, sh32, [&](uint32_t e, uint32_t pre, uint32_t) {
+            base[e] = pre;
+            if (keep) keep[e] = pre;
+        });
         if (threadIdx.x == 0) {
             base[nb] = total;
             if (keep) keep[nb] = total;
@@ -400,14 +404,20 @@ __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
     // map and the queue totals are written here.
     const bool one = a.n_joins == 1;
     const unsigned long long map_cap = a.taskmap_cap;
-    const unsigned long long total = block_scan_spans<unsigned long long>(
-        n, sh64,
-        [&](uint32_t e, bool ok) {
-            const uint32_t i = ok ? e : 0u;
-            const uint32_t r0 = rb[i], r1 = rb[i + 1], q0 = sb[i], q1 = sb[i + 1];
-            return ok ? part_tasks(r1 - r0, q1 - q0, jd) : 0ull;
-        },
-        [&](uint32_t e, unsigned long long pre, unsigned long long t) {
+    const Spans sp(n);
+    // partition sizes: every base is loaded before the first division
+    uint32_t nr[kScanRounds], ns[kScanRounds];
+#pragma unroll
+    for (int j = 0; j < kScanRounds; j++) {
+        const uint32_t i = sp.ok(j) ? sp.elem(j) : 0u;
+        nr[j] = rb[i + 1] - rb[i];
+        ns[j] = sb[i + 1] - sb[i];
+    }
+    unsigned long long v[kScanRounds];
+#pragma unroll
+    for (int j = 0; j < kScanRounds; j++) v[j] = sp.ok(j) ? part_tasks(nr[j], ns[j], jd) : 0ull;
+    const unsigned long long total =
+        block_scan_spans<unsigned long long>(sp, v, sh64, [&](uint32_t e, unsigned long long pre, unsigned long long t) {
             pt[e] = pre;
             if (one)
                 for (unsigned long long q = pre; q < pre + t && q < map_cap; q++) tmap[q] = e;

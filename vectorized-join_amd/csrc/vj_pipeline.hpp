@@ -36,6 +36,7 @@ constexpr int kPassWaves = kPassThreads / 64;
 constexpr int kHistThreads = 1024;  // histogram kernel
 constexpr int kProbeThreads = 512;  // build+probe kernel
 constexpr int kProbeWaves = kProbeThreads / 64;
+constexpr int kMaxSegments = 512;  // segments (histogram blocks) per job
 constexpr int kMaxPassBits = 8;     // digits per pass <= 256
 constexpr int kMaxHistBits = 14;    // 2D histogram in LDS: 2^14 x 4 B = 64 KiB
 constexpr uint32_t kXcds = 8;       // accelerator dies (XCDs) of an MI355X

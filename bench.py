@@ -119,9 +119,9 @@ def traffic_from_pmc(fetch_csv, write_csv, workload):
     lines = ["# HBM traffic per launch, from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch),",
              "# each scaled by counter(1 GiB device copy) / 1 GiB. Workload: " + workload,
              "# made by: rocprofv3 --pmc FETCH_SIZE -d F -o run -- python3 bench.py --steps 1 --warmup 1 --no-cpu "
-             "--no-batched --no-serving --no-tune --calib-copy",
+             "--no-batched --no-serving --no-tune --no-multicol --calib-copy",
              "#          rocprofv3 --pmc WRITE_SIZE -d W -o run -- python3 bench.py --steps 1 --warmup 1 --no-cpu "
-             "--no-batched --no-serving --no-tune --calib-copy",
+             "--no-batched --no-serving --no-tune --no-multicol --calib-copy",
              "#          python3 bench.py --traffic-from F/run_results.db W/run_results.db"]
     factors, per = {}, {}
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
@@ -568,8 +568,9 @@ def main():
         "dtype": "int64",
         "data": "synthetic (seeded: workloads.large_fk)",
         "config": {"workload": "large_fk: build 2^%d unique int64 keys (mix64 of a permutation), probe 2^%d FK "
-                               "keys (100%% match), FNV-1a-64, %d radix passes, bits autotuned" % (
-                                  args.log_r, args.log_s, summ["passes"]),
+                               "keys (100%% match), FNV-1a-64, %d radix passes, bits %s" % (
+                                  args.log_r, args.log_s, summ["passes"],
+                                  "of the planner" if args.no_tune else "autotuned"),
                    "build_rows": nr, "probe_rows": ns, "matches": m, "passes": summ["passes"],
                    "bits": [summ["bits1"], summ["bits2"]], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "%s (%s)" % (STAGE_KERNEL[dominant], dominant),

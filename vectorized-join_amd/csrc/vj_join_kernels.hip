@@ -500,13 +500,18 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>()))
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (w == 0) {
-        // Workgroups are dispatched round-robin over the XCDs; give each XCD a
-        // contiguous range of tiles so that the runs adjacent tiles write to
-        // one partition (which share cache lines) meet in the same L2.
+        // Workgroups are dispatched round-robin over the XCDs. Consecutive
+        // tiles go to one XCD, so that the runs adjacent tiles write to one
+        // partition (which share cache lines) meet in the same L2: pass 1
+        // gives each XCD one contiguous range of tiles; pass 2 deals out
+        // groups of kXcdGroup consecutive chunks round-robin, so that a run of
+        // heavy chunks (a skewed partition) is spread over all XCDs (block k
+        // of XCD x takes chunk (k / G * 8 + x) * G + k % G).
         const uint32_t *pfx = pass == 1 ? a.p1_tile0 : a.p2_chunk0;
         const uint32_t total = pass == 1 ? a.total_p1_tiles : a.total_p2_chunks;
-        const uint32_t per_xcd = (total + kXcds - 1) / kXcds;
-        const uint32_t b = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+        const uint32_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
+        const uint32_t b = pass == 1 ? x * ((total + kXcds - 1) / kXcds) + k
+                                     : ((k / kXcdGroup) * kXcds + x) * kXcdGroup + k % kXcdGroup;
         const uint32_t valid = b < total;
         const uint32_t j = valid ? wave_upper_index(pfx, a.n_jobs, b) : 0;
         if (lane == 0) {
@@ -1168,7 +1173,8 @@ hipError_t launch_reset_probe(const PipeArgs &a, hipStream_t s) {
 hipError_t launch_pass(const PipeArgs &a, const LaunchShape &sh, int pass, hipStream_t s) {
     const uint32_t total = pass == 1 ? a.total_p1_tiles : a.total_p2_chunks;
     if (total == 0) return hipSuccess;
-    const uint32_t grid = (total + kXcds - 1) / kXcds * kXcds;
+    const uint32_t groups = (total + kXcdGroup - 1) / kXcdGroup;  // pass 2: every XCD gets the same number of groups
+    const uint32_t grid = pass == 1 ? (total + kXcds - 1) / kXcds * kXcds : (groups + kXcds - 1) / kXcds * kXcds * kXcdGroup;
     const size_t lds = pass_lds_bytes(sh.kw, sh.nc);
     return dispatch(sh, [&]<int HP, int KW, int NC>() {
         hipLaunchKernelGGL((k_pass<HP, KW, NC>), dim3(grid), dim3(kPassThreads), lds, s, a, pass);

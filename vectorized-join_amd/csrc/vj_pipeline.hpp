@@ -40,7 +40,7 @@ constexpr int kMaxSegments = 512;  // segments (histogram blocks) per job
 constexpr int kMaxPassBits = 8;     // digits per pass <= 256
 constexpr int kMaxHistBits = 14;    // 2D histogram in LDS: 2^14 x 4 B = 64 KiB
 constexpr uint32_t kXcds = 8;       // accelerator dies (XCDs) of an MI355X
-constexpr uint32_t kXcdGroup = 16;  // consecutive pass-2 chunks kept on one XCD
+constexpr uint32_t kXcdGroup = 32;  // consecutive pass-2 chunks kept on one XCD
 
 // Tuples per pass tile, per build chunk and per probe chunk, by key bytes per row.
 __host__ __device__ constexpr int pass_items(int key_bytes) { return key_bytes <= 8 ? 6 : (key_bytes <= 16 ? 4 : 2); }

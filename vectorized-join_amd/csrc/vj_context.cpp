@@ -120,6 +120,10 @@ struct Plan {
     uint32_t max_nparts = 0;
     uint32_t max_hist_bins = 1, total_hist_blocks = 0, total_p1_tiles = 0, total_p2_chunks = 0;
     bool any_part = false, any_p2 = false;
+    // histogram blocks wanted over the whole pipeline (two per CU) and the rows
+    // they share; a 2-pass job gets about its share of the blocks
+    uint64_t hist_rows_total = 0;
+    uint32_t hist_blocks_target = 0;
     LaunchShape shape{};
     HashCfg hash{};
     int kb = 0;
@@ -246,25 +250,36 @@ void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0
         const uint64_t chunk = (uint64_t)T * 15 / 16;
         uint64_t want = jd.b2 ? ((uint64_t)jd.n + (uint64_t)n1 * chunk - 1) / ((uint64_t)n1 * chunk)
                               : ((uint64_t)jd.p1_tiles + 15) / 16;
-        want = std::max<uint64_t>(1, std::min<uint64_t>({want, kMaxSegments, jd.p1_tiles}));
+        want = std::max<uint64_t>(1, std::min<uint64_t>({want, (uint64_t)kMaxSegments, jd.p1_tiles}));
         jd.seg_tiles = (uint32_t)((jd.p1_tiles + want - 1) / want);
         jd.nseg = (jd.p1_tiles + jd.seg_tiles - 1) / jd.seg_tiles;
+        // 2-pass jobs with fewer segments than their share of the histogram
+        // blocks split each segment into hsplit histogram rows
+        uint64_t hs = 1;
+        if (jd.b2 && pl.hist_rows_total && pl.hist_blocks_target) {
+            const uint64_t share = (uint64_t)pl.hist_blocks_target * jd.n / pl.hist_rows_total;
+            hs = std::max<uint64_t>(1, std::min<uint64_t>({share / jd.nseg, (uint64_t)kMaxSegments / jd.nseg,
+                                                           (uint64_t)jd.seg_tiles}));
+        }
+        jd.hsplit = (uint32_t)hs;
+        jd.hrow_tiles = (jd.seg_tiles + jd.hsplit - 1) / jd.hsplit;
+        jd.nrow = jd.nseg * jd.hsplit;
         jd.tilepre_off = pl.counts_words;
         pl.counts_words += (uint64_t)jd.p1_tiles * n1;
         jd.m1_off = pl.counts_words;
-        pl.counts_words += (uint64_t)(jd.nseg + 1) * n1;
+        pl.counts_words += (uint64_t)(jd.nrow + 1) * n1;
         for (uint32_t c0 = 0; c0 < n1; c0 += kColScanCols) pl.colscan.push_back({job_index, 1u, c0, 0u});
         if (jd.b2) {
             pl.any_p2 = true;
             jd.m2_off = pl.counts_words;
-            pl.counts_words += (uint64_t)(jd.nseg + 1) * nb;
+            pl.counts_words += (uint64_t)(jd.nrow + 1) * nb;
             for (uint32_t c0 = 0; c0 < nb; c0 += kColScanCols) pl.colscan.push_back({job_index, 2u, c0, 0u});
             chunks = n1 * jd.nseg;
         }
         pl.max_hist_bins = std::max(pl.max_hist_bins, (jd.b2 ? nb : 0) + n1 * (1 + kHistThreads / 64));
     }
     pl.hist_block0.push_back(pl.total_hist_blocks);
-    pl.total_hist_blocks += jd.nseg;
+    pl.total_hist_blocks += jd.nrow;
     pl.p1_tile0.push_back(pl.total_p1_tiles);
     pl.total_p1_tiles += jd.p1_tiles;
     pl.p2_chunk0.push_back(pl.total_p2_chunks);
@@ -336,6 +351,8 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
     pl.shape.hp = hash_path(pl.hash);
     pl.shape.kw = t0->key_width;
     pl.shape.nc = t0->n_cols;
+    for (const JoinSpec &js : specs) pl.hist_rows_total += (js.res ? 0 : js.r->n_rows) + js.s->n_rows;
+    pl.hist_blocks_target = 2u * (uint32_t)(c ? c->cus : 256);  // sizing without a context: an MI355X
     for (size_t i = 0; i < specs.size(); i++) {
         const JoinSpec &js = specs[i];
         int P, b1, b2;

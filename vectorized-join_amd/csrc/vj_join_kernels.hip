@@ -234,8 +234,14 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(PipeArgs a) {
     const uint32_t m1 = n1 - 1, m2 = (1u << job.b2) - 1, b2 = job.b2;
     const uint32_t s1 = job.shift0, s2 = job.shift0 + job.b1;
     const HashCfg hc = a.hash;
-    const uint32_t t_begin = g * job.seg_tiles;
-    const uint32_t t_end = t_begin + job.seg_tiles < job.p1_tiles ? t_begin + job.seg_tiles : job.p1_tiles;
+    // histogram row g: the hrow_tiles tiles of segment g / hsplit starting at
+    // its (g % hsplit)-th row (the last rows of a segment may be short or empty)
+    const uint32_t sg = g / job.hsplit;
+    const uint32_t s_end0 = (sg + 1) * job.seg_tiles;
+    const uint32_t s_end = s_end0 < job.p1_tiles ? s_end0 : job.p1_tiles;
+    const uint32_t r0 = sg * job.seg_tiles + (g - sg * job.hsplit) * job.hrow_tiles;
+    const uint32_t t_begin = r0 < s_end ? r0 : s_end;
+    const uint32_t t_end = t_begin + job.hrow_tiles < s_end ? t_begin + job.hrow_tiles : s_end;
     const gptr<uint32_t> tilepre = to_global(a.counts + job.tilepre_off);
     const GKeys<KW, NC> in = uniform_keys<KW, NC>(job.in);
     const uint32_t n = job.n;
@@ -281,7 +287,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(PipeArgs a) {
     }
 }
 
-// Exclusive scans down kColScanCols columns of a [nseg + 1][ncols] count
+// Exclusive scans down kColScanCols columns of a [nrow + 1][ncols] count
 // matrix (the last row receives the column totals). 1024 threads = 64 columns
 // x 16 row groups: each thread reads its rows once into registers (all loads
 // in flight together), the groups' sums are combined in LDS, and the rows are
@@ -296,7 +302,7 @@ __global__ __launch_bounds__(1024) void k_colscan(PipeArgs a) {
     const gptr<uint32_t> m = to_global(a.counts + (d.which == 1 ? job.m1_off : job.m2_off));
     const uint32_t cc = threadIdx.x % kColScanCols, rg = threadIdx.x / kColScanCols;
     const uint32_t c = d.col0 + cc;
-    const uint32_t rows = job.nseg, per = (rows + kColScanGroups - 1) / kColScanGroups;  // <= kColScanRows
+    const uint32_t rows = job.nrow, per = (rows + kColScanGroups - 1) / kColScanGroups;  // <= kColScanRows
     const uint32_t lo = rg * per < rows ? rg * per : rows, hi = lo + per < rows ? lo + per : rows;
     const bool col_ok = c < ncols;
     const uint32_t cl = col_ok ? c : 0u;
@@ -366,7 +372,7 @@ __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
             if (keep) keep[q] = 0;
         }
     } else {
-        const gptr<const uint32_t> tot = to_global(a.counts + (job.b2 ? job.m2_off : job.m1_off) + (uint64_t)job.nseg * nb);
+        const gptr<const uint32_t> tot = to_global(a.counts + (job.b2 ? job.m2_off : job.m1_off) + (uint64_t)job.nrow * nb);
         const Spans sp(nb);  // nb <= 2^14
         uint32_t v[kScanRounds];
 #pragma unroll
@@ -539,8 +545,8 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>()))
                     ti.src.n = job.n;
                     for (int c = 0; c < 4; c++) ti.dst_cols[c] = job.fin_cols[c];
                     ti.dst_ids = job.fin_ids;
-                    ti.lo = p + m1[(uint64_t)g * n1 + d1];
-                    ti.hi = p + m1[(uint64_t)(g + 1) * n1 + d1];
+                    ti.lo = p + m1[(uint64_t)g * job.hsplit * n1 + d1];
+                    ti.hi = p + m1[(uint64_t)(g + 1) * job.hsplit * n1 + d1];
                     ti.bins = 1u << job.b2;
                     ti.shift = job.shift0 + job.b1;
                 }
@@ -553,14 +559,15 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>()))
         const JobDesc &job = a.jobs[ti.job];
         const uint32_t *base = a.base + job.base_off;
         if (pass == 1) {
-            const uint32_t n1 = 1u << job.b1, t = ti.local, g = t / job.seg_tiles;
+            const uint32_t n1 = 1u << job.b1, t = ti.local, sg = t / job.seg_tiles;
+            const uint32_t g = sg * job.hsplit + (t - sg * job.seg_tiles) / job.hrow_tiles;  // histogram row
             gdst[tid] = base[tid << job.b2] + a.counts[job.m1_off + (uint64_t)g * n1 + tid] +
                         a.counts[job.tilepre_off + (uint64_t)t * n1 + tid];
         } else {
             const uint32_t nb = 1u << (job.b1 + job.b2);
             const uint32_t d1 = ti.local / job.nseg, g = ti.local - d1 * job.nseg;
             const uint32_t q = (d1 << job.b2) | tid;
-            gdst[tid] = base[q] + a.counts[job.m2_off + (uint64_t)g * nb + q];
+            gdst[tid] = base[q] + a.counts[job.m2_off + (uint64_t)g * job.hsplit * nb + q];
         }
     }
 

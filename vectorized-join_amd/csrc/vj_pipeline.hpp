@@ -7,18 +7,21 @@
 //
 // Partitioning is count-then-scatter with exact destinations, so scatter
 // blocks never wait on each other:
-//  * A job's rows are cut into pass-1 tiles (T rows) grouped into segments.
-//    The histogram kernel walks each segment's tiles in order and records, per
-//    tile, the exclusive prefix of its pass-1 digit counts inside the segment
-//    (tilepre), and per segment the pass-1 digit counts (m1 row) and the
+//  * A job's rows are cut into pass-1 tiles (T rows) grouped into segments,
+//    and each segment into hsplit histogram rows of consecutive tiles (more
+//    histogram blocks than segments when the segments alone would leave CUs
+//    idle). The histogram kernel walks each row's tiles in order and records,
+//    per tile, the exclusive prefix of its pass-1 digit counts inside the row
+//    (tilepre), and per row the pass-1 digit counts (m1 row) and the
 //    (pass-1, pass-2) digit counts (m2 row).
-//  * Column scans over the segments turn m1 / m2 into per-segment offsets,
-//    and a scan over the column totals gives the partition bases.
+//  * Column scans over the rows turn m1 / m2 into per-row offsets, and a scan
+//    over the column totals gives the partition bases.
 //  * Pass 1 writes tile t's rows of digit d1 at
-//      base[d1 << b2] + m1[seg(t)][d1] + tilepre[t][d1] + rank.
+//      base[d1 << b2] + m1[row(t)][d1] + tilepre[t][d1] + rank.
 //  * Pass 2 reads chunk (d1, g) = segment g's rows of pass-1 partition d1 (a
-//    contiguous range of the pass-1 output) and writes its rows of digit d2 at
-//      base[(d1 << b2) | d2] + m2[g][(d1 << b2) | d2] + rank.
+//    contiguous range of the pass-1 output, from row g * hsplit to the next
+//    segment's) and writes its rows of digit d2 at
+//      base[(d1 << b2) | d2] + m2[g * hsplit][(d1 << b2) | d2] + rank.
 //
 // HBM layout per partitioned job: SoA copies of the key columns plus a uint32
 // row-id column, once after pass 1 ("mid", only for 2-pass jobs) and once
@@ -60,10 +63,14 @@ struct JobDesc {
     uint32_t base_off;        // u32 offset of the 2^(b1+b2)+1 partition bases in the base arena
     uint32_t p1_tiles;        // pass-1 tiles
     uint32_t seg_tiles;       // pass-1 tiles per segment
-    uint32_t nseg;            // segments (= histogram blocks)
+    uint32_t nseg;            // segments (pass-2 chunks of one pass-1 digit)
+    uint32_t hsplit;          // histogram rows (blocks) per segment
+    uint32_t hrow_tiles;      // pass-1 tiles per histogram row (ceil(seg_tiles / hsplit))
+    uint32_t nrow;            // histogram rows = nseg * hsplit (= histogram blocks)
+    uint32_t pad_;
     uint64_t tilepre_off;     // u32 offset of tilepre [p1_tiles][2^b1] in the count arena
-    uint64_t m1_off;          // u32 offset of m1 [nseg + 1][2^b1]
-    uint64_t m2_off;          // u32 offset of m2 [nseg + 1][2^(b1+b2)] (2-pass jobs)
+    uint64_t m1_off;          // u32 offset of m1 [nrow + 1][2^b1]
+    uint64_t m2_off;          // u32 offset of m2 [nrow + 1][2^(b1+b2)] (2-pass jobs)
     // Resident build sides. res_base != null: the job is already partitioned
     // (fin_* point at a resident table, res_base at its 2^(b1+b2)+1 bases) and
     // no hist / pass work is scheduled for it. keep_base != null: the bases
@@ -104,7 +111,7 @@ struct PipeArgs {
     const JobDesc *jobs;
     const JoinDesc *joins;
     const ColScanDesc *colscan;     // [n_colscan]
-    const uint32_t *hist_block0;    // [n_jobs + 1] prefix of nseg (host)
+    const uint32_t *hist_block0;    // [n_jobs + 1] prefix of nrow (host)
     const uint32_t *p1_tile0;       // [n_jobs + 1] prefix of p1_tiles (host)
     const uint32_t *p2_chunk0;      // [n_jobs + 1] prefix of pass-2 chunks (host)
     unsigned long long *join_tasks;   // [n_joins] (device)

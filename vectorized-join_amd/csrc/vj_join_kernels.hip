@@ -820,7 +820,7 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const uint64_t h = hash_key<HP, KW, NC>(sk[u], hc);
-            st[u] = (((uint32_t)(h >> tag_shift) & 0xFFFFu) << 16) | ((uint32_t)(h >> shift) & bmask);
+            st[u] = ((((uint32_t)(h >> tag_shift) & 0xFFFFu) | 1u) << 16) | ((uint32_t)(h >> shift) & bmask);
         }
         while (__ballot(act != 0)) {
             uint4 E[U];
@@ -831,22 +831,31 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
                 const bool live = (act >> u) & 1u;
                 const uint32_t tg = st[u] >> 16;
                 const uint32_t e4[kBucketSlots] = {E[u].x, E[u].y, E[u].z, E[u].w};
-                uint32_t cand = 0;  // entries whose tag matches, in slot order
-                bool emp = false;   // the bucket has a free slot: the row's key cannot be further on
+                // entries whose tag matches, in slot order (tags are never 0, so an
+                // empty slot never matches); a bucket's slots fill in order, so it
+                // has a free slot (the row's key cannot be further on) iff its last
+                // slot is free
+                uint32_t cand = 0;
 #pragma unroll
-                for (int j = 0; j < kBucketSlots; j++) {
-                    cand |= (e4[j] != 0 && (e4[j] >> 16) == tg ? 1u : 0u) << j;
-                    emp |= e4[j] == 0;
-                }
+                for (int j = 0; j < kBucketSlots; j++) cand |= ((e4[j] >> 16) == tg ? 1u : 0u) << j;
+                const bool emp = e4[kBucketSlots - 1] == 0;
                 if (!live) cand = 0;
-                bool found = false;
-                uint32_t fo = 0;
-                while (__ballot(cand != 0)) {  // nearly always one pass
+                // the first candidate, branch-free (a row has at most one entry of
+                // its key; a second candidate means a 16-bit tag collision)
+                uint32_t ej = e4[kBucketSlots - 1];
+#pragma unroll
+                for (int j = kBucketSlots - 2; j >= 0; j--) ej = (cand >> j) & 1u ? e4[j] : ej;
+                uint32_t fo = cand ? (ej & 0xFFFFu) - 1 : 0u;
+                bool found = cand != 0;
+#pragma unroll
+                for (int c = 0; c < NC; c++) found &= rk[c * CAP + fo] == sk[u].c[c];
+                cand = found ? 0u : cand & (cand - 1);
+                while (__ballot(cand != 0)) {  // the other candidates, in slot order
                     const bool has = cand != 0;
                     const uint32_t j = __builtin_ctz(cand | (1u << kBucketSlots));
-                    const uint32_t ej = j == 0 ? e4[0] : (j == 1 ? e4[1] : (j == 2 ? e4[2] : e4[3]));
+                    const uint32_t ejj = j == 0 ? e4[0] : (j == 1 ? e4[1] : (j == 2 ? e4[2] : e4[3]));
                     cand &= cand - 1;
-                    const uint32_t o = has ? (ej & 0xFFFFu) - 1 : 0u;
+                    const uint32_t o = has ? (ejj & 0xFFFFu) - 1 : 0u;
                     bool eq = has;
 #pragma unroll
                     for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == sk[u].c[c];
@@ -1050,7 +1059,7 @@ __global__ __launch_bounds__(kProbeThreads, (probe_waves_per_simd<HP, KW * NC>()
 #pragma unroll
             for (int c = 0; c < NC; c++) k.c[c] = rk[c * CAP + i];
             const uint64_t h = hash_key<HP, KW, NC>(k, hc);
-            const uint32_t tag = (uint32_t)(h >> tag_shift) & 0xFFFFu;
+            const uint32_t tag = ((uint32_t)(h >> tag_shift) & 0xFFFFu) | 1u;  // never 0: an entry is never 0
             const uint32_t e = (tag << 16) | (i + 1);
             uint32_t b = (uint32_t)(h >> shift) & bmask;
             // Scan slots in order (the bucket, then the next buckets). An empty

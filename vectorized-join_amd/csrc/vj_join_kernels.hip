@@ -897,6 +897,9 @@ __device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint16_t *
 #pragma unroll
     for (int g = 0; g < kProbeIters; g += G) {
         if (walk_wave_row<U>(wa, g) >= s_end) break;  // wave-uniform
+        // ids of every row, matched or not, with clamped indexes: no per-row
+        // branches, and the loads of a group are issued together (rows past
+        // the chunk hold stale results: they read row s_lo / build row 0)
         uint32_t sid[G * U], bid[G * U];
 #pragma unroll
         for (int it = 0; it < G; it++) {
@@ -904,14 +907,10 @@ __device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint16_t *
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const uint32_t i = row + u * 64;
+                const bool in = i < s_end;
                 const uint32_t o = res_get(res, (g + it) * U + u);
-                const bool m = i < s_end && o != kNoMatch;
-                sid[it * U + u] = i;
-                bid[it * U + u] = 0;
-                if (m) {
-                    if (wa.s.rowids) sid[it * U + u] = wa.s.rowids[i];
-                    bid[it * U + u] = build_id(wa, o);
-                }
+                sid[it * U + u] = wa.s.rowids ? wa.s.rowids[in ? i : wa.s_lo] : i;  // wave-uniform test
+                bid[it * U + u] = build_id(wa, in && o != kNoMatch ? o : 0u);
             }
         }
 #pragma unroll

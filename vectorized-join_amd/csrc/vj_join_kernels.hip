@@ -831,38 +831,38 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
                 const bool live = (act >> u) & 1u;
                 const uint32_t tg = st[u] >> 16;
                 const uint32_t e4[kBucketSlots] = {E[u].x, E[u].y, E[u].z, E[u].w};
-                // entries whose tag matches, in slot order (tags are never 0, so an
-                // empty slot never matches); a bucket's slots fill in order, so it
-                // has a free slot (the row's key cannot be further on) iff its last
-                // slot is free
-                uint32_t cand = 0;
+                // the first entry whose tag matches (0: none; tags are never 0, so
+                // neither an entry nor a match is 0); a bucket's slots fill in
+                // order, so it has a free slot (the row's key cannot be further
+                // on) iff its last slot is free
+                uint32_t ej = 0;
 #pragma unroll
-                for (int j = 0; j < kBucketSlots; j++) cand |= ((e4[j] >> 16) == tg ? 1u : 0u) << j;
+                for (int j = kBucketSlots - 1; j >= 0; j--) ej = (e4[j] >> 16) == tg ? e4[j] : ej;
                 const bool emp = e4[kBucketSlots - 1] == 0;
-                if (!live) cand = 0;
-                // the first candidate, branch-free (a row has at most one entry of
-                // its key; a second candidate means a 16-bit tag collision)
-                uint32_t ej = e4[kBucketSlots - 1];
-#pragma unroll
-                for (int j = kBucketSlots - 2; j >= 0; j--) ej = (cand >> j) & 1u ? e4[j] : ej;
-                uint32_t fo = cand ? (ej & 0xFFFFu) - 1 : 0u;
-                bool found = cand != 0;
+                if (!live) ej = 0;
+                uint32_t fo = ej ? (ej & 0xFFFFu) - 1 : 0u;
+                bool found = ej != 0;
 #pragma unroll
                 for (int c = 0; c < NC; c++) found &= rk[c * CAP + fo] == sk[u].c[c];
-                cand = found ? 0u : cand & (cand - 1);
-                while (__ballot(cand != 0)) {  // the other candidates, in slot order
-                    const bool has = cand != 0;
-                    const uint32_t j = __builtin_ctz(cand | (1u << kBucketSlots));
-                    const uint32_t ejj = j == 0 ? e4[0] : (j == 1 ? e4[1] : (j == 2 ? e4[2] : e4[3]));
-                    cand &= cand - 1;
-                    const uint32_t o = has ? (ejj & 0xFFFFu) - 1 : 0u;
-                    bool eq = has;
+                if (__ballot(ej != 0 && !found)) {  // a 16-bit tag collision: the later candidates, in slot order
+                    uint32_t cand = 0;
 #pragma unroll
-                    for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == sk[u].c[c];
-                    if (eq) {
-                        found = true;
-                        fo = o;
-                        cand = 0;
+                    for (int j = 0; j < kBucketSlots; j++) cand |= ((e4[j] >> 16) == tg ? 1u : 0u) << j;
+                    cand = ej != 0 && !found ? cand & (cand - 1) : 0u;
+                    while (__ballot(cand != 0)) {
+                        const bool has = cand != 0;
+                        const uint32_t j = __builtin_ctz(cand | (1u << kBucketSlots));
+                        const uint32_t ejj = j == 0 ? e4[0] : (j == 1 ? e4[1] : (j == 2 ? e4[2] : e4[3]));
+                        cand &= cand - 1;
+                        const uint32_t o = has ? (ejj & 0xFFFFu) - 1 : 0u;
+                        bool eq = has;
+#pragma unroll
+                        for (int c = 0; c < NC; c++) eq &= rk[c * CAP + o] == sk[u].c[c];
+                        if (eq) {
+                            found = true;
+                            fo = o;
+                            cand = 0;
+                        }
                     }
                 }
                 const uint32_t cur = res_get(res, it * U + u);

@@ -477,8 +477,8 @@ def main():
         print(text, end="")
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; the join shards by a top radix digit across GPUs "
-              "(see DESIGN.md) but no multi-GPU run exists. Use --gpus 1.", file=sys.stderr)
+        print("bench.py: only one GPU is measured; across GPUs the join would shard by a top radix digit "
+              "of the hash, but no multi-GPU path exists. Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

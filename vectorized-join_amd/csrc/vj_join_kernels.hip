@@ -393,8 +393,10 @@ __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
 
 // One block of 1024 threads per join: build/probe tasks = R chunks x S chunks
 // of every partition pair, as an exclusive prefix over partitions.
-__device__ __forceinline__ unsigned long long part_tasks(uint32_t nr, uint32_t ns, const JoinDesc &jd) {
-    return nr && ns ? (unsigned long long)((nr + jd.cap_r - 1) / jd.cap_r) * ((ns + jd.cap_s - 1) / jd.cap_s) : 0ull;
+// The chunk capacities are powers of two (cap = 1 << log2), so the chunk
+// counts are shifts.
+__device__ __forceinline__ unsigned long long part_tasks(uint32_t nr, uint32_t ns, uint32_t lr, uint32_t ls) {
+    return nr && ns ? (unsigned long long)(((nr - 1) >> lr) + 1) * (((ns - 1) >> ls) + 1) : 0ull;
 }
 
 __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
@@ -420,8 +422,9 @@ __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
         ns[j] = sb[i + 1] - sb[i];
     }
     unsigned long long v[kScanRounds];
+    const uint32_t lr = (uint32_t)__builtin_ctz(jd.cap_r), ls = (uint32_t)__builtin_ctz(jd.cap_s);
 #pragma unroll
-    for (int j = 0; j < kScanRounds; j++) v[j] = sp.ok(j) ? part_tasks(nr[j], ns[j], jd) : 0ull;
+    for (int j = 0; j < kScanRounds; j++) v[j] = sp.ok(j) ? part_tasks(nr[j], ns[j], lr, ls) : 0ull;
     const unsigned long long total =
         block_scan_spans<unsigned long long>(sp, v, sh64, [&](uint32_t e, unsigned long long pre, unsigned long long t) {
             pt[e] = pre;

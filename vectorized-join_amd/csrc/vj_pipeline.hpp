@@ -50,6 +50,9 @@ __host__ __device__ constexpr int pass_items(int key_bytes) { return key_bytes <
 __host__ __device__ constexpr int pass_tile(int key_bytes) { return kPassThreads * pass_items(key_bytes); }
 __host__ __device__ constexpr int build_cap(int key_bytes) { return key_bytes <= 8 ? 2048 : 1024; }
 constexpr int kProbeChunk = 16384;  // probe rows per build+probe task
+static_assert((kProbeChunk & (kProbeChunk - 1)) == 0 && (build_cap(4) & (build_cap(4) - 1)) == 0 &&
+                  (build_cap(8) & (build_cap(8) - 1)) == 0 && (build_cap(16) & (build_cap(16) - 1)) == 0,
+              "chunk capacities are powers of two (the task counts are shifts)");
 
 struct JobDesc {
     TView in;                 // the caller's table
@@ -83,7 +86,7 @@ struct JoinDesc {
     uint32_t job_r, job_s;
     uint32_t *out_r, *out_s;
     uint64_t cap;             // output capacity (pairs)
-    uint32_t cap_r, cap_s;    // tuples per build chunk / probe chunk
+    uint32_t cap_r, cap_s;    // tuples per build chunk / probe chunk (powers of two)
     uint32_t ptask_off;       // u64 offset of the per-partition task prefix (nparts + 1)
     uint32_t nparts;          // 2^(b1+b2)
     uint32_t slot_shift;      // hash bits consumed by partitioning (slot uses h >> slot_shift)

@@ -128,6 +128,17 @@ def test_zipf_skewed_multicolumn_sample(ctx):
     _assert_same(got, want)
 
 
+@pytest.mark.parametrize("kind", ["fnv", "custom_xor_hw"])
+def test_zipf_multicolumn_two_passes_mid_size(ctx, kind):
+    # build 2^17 rows of 4 x u32 keys, probe 2^20 (80 % Zipf): 2048-row tiles leave few segments per
+    # pass-1 digit, so the planner splits each segment over several histogram blocks, and the Zipf
+    # head makes a run of heavy pass-2 chunks that the XCD groups spread out
+    rcols, scols = workloads.multicol_skew(log_distinct=15, log_s=20)
+    got = _pairs(*_run(ctx, rcols, scols, kind=kind, passes=2))
+    want = np_join(workloads.reference_rows(rcols), workloads.reference_rows(scols))
+    _assert_same(got, want)
+
+
 @pytest.mark.parametrize("nr,ns", [(0, 100), (100, 0), (0, 0), (1, 1), (5, 70000)])
 def test_empty_and_tiny(ctx, nr, ns):
     import torch

@@ -14,7 +14,9 @@
  *  - All table and output pointers are device pointers (HBM) unless stated.
  *  - `stream` is a hipStream_t passed as void*; NULL is the default stream.
  *  - A vj_ctx is not thread-safe; distinct contexts are independent.
- *  - Tables hold fewer than 2^32 rows; row ids are uint32.
+ *  - Tables hold at most VJ_MAX_ROWS rows; row ids are uint32. (The kernels
+ *    index rows, tiles and probe chunks in 32-bit arithmetic; the margin below
+ *    2^32 keeps the last tile's and chunk's end inside that range.)
  */
 #ifndef VJ_H
 #define VJ_H
@@ -26,8 +28,9 @@
 extern "C" {
 #endif
 
-#define VJ_ABI_VERSION 1
+#define VJ_ABI_VERSION 2
 #define VJ_MAX_KEY_COLS 4
+#define VJ_MAX_ROWS 4294901760ull /* 2^32 - 2^16 */
 
 typedef enum vj_status {
     VJ_OK = 0,
@@ -192,6 +195,15 @@ typedef struct vj_join_summary {
     uint64_t scratch_bytes;
     int32_t graph_replay;  /* 1: the join ran as a replay of a captured hipGraph (same launch shape as before) */
     int32_t reserved;
+    /* Algorithmic HBM bytes of each stage -- the key bytes and row ids of every
+     * tuple the stage reads and writes, plus 8 bytes per output pair -- and the
+     * resulting GB/s over the stage's device time (0 while per-stage timing is
+     * off; the total is always timed). The reference reports per-kernel GB/s in
+     * PartitionSummary / ProbeSummary (src/join/join_provider.cu:37-93) and
+     * per-phase tuples/s in JoinSummary (join_provider.cuh:54-106). */
+    uint64_t bytes_histogram, bytes_pass1, bytes_pass2, bytes_build_probe, bytes_total;
+    double gbps_histogram, gbps_pass1, gbps_pass2, gbps_build_probe, gbps_total;
+    double probe_tuples_per_s; /* probe rows / ms_total */
 } vj_join_summary;
 
 vj_status vj_join_summary_get(vj_ctx *ctx, vj_join_summary *summary);

@@ -68,6 +68,10 @@ def _empty_table(n_cols=1, key_width=8, n_rows=0):
     (lambda t, o: setattr(o, "chunk_bits", 16), "VJ_ERR_INVALID_ARGUMENT"),
     (lambda t, o: setattr(o, "chunk_bits", 128), "VJ_ERR_INVALID_ARGUMENT"),  # 1 column = 8 bytes < 16-byte chunk
     (lambda t, o: setattr(t, "n_rows", 1 << 32), "VJ_ERR_UNSUPPORTED"),
+    # one row past the limit is refused before anything else is looked at
+    (lambda t, o: setattr(t, "n_rows", (1 << 32) - (1 << 16) + 1), "VJ_ERR_UNSUPPORTED"),
+    # at the limit the row count is accepted; the NULL key column is what fails
+    (lambda t, o: setattr(t, "n_rows", (1 << 32) - (1 << 16)), "VJ_ERR_INVALID_ARGUMENT"),
 ])
 def test_hash_argument_errors(mutate, code):
     t = _empty_table()
@@ -85,3 +89,9 @@ def test_hash_empty_table_is_ok_without_device():
     assert (o.kind, o.hash_bits, o.chunk_bits, o.word_mode) == (0, 64, 64, 0)
     assert vjoin.lib().vj_hash(ctypes.byref(t), ctypes.byref(o), None, None) == 0
     assert vjoin.lib().vj_hash(None, ctypes.byref(o), None, None) == vjoin.STATUS_CODE["VJ_ERR_INVALID_ARGUMENT"]
+
+
+def test_row_limit_matches_header():
+    text = open(HEADER).read()
+    m = re.search(r"#define VJ_MAX_ROWS (\d+)ull", text)
+    assert m and int(m.group(1)) == (1 << 32) - (1 << 16)

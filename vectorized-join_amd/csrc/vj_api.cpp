@@ -33,7 +33,11 @@ Err check_table(const vj_table *t, bool) {
     if (!t) return Err::make(VJ_ERR_INVALID_ARGUMENT, "table is NULL");
     if (t->n_cols < 1 || t->n_cols > VJ_MAX_KEY_COLS) return Err::make(VJ_ERR_INVALID_ARGUMENT, "n_cols must be 1..4");
     if (t->key_width != 4 && t->key_width != 8) return Err::make(VJ_ERR_INVALID_ARGUMENT, "key_width must be 4 or 8");
-    if (t->n_rows >= (1ull << 32)) return Err::make(VJ_ERR_UNSUPPORTED, "tables must have fewer than 2^32 rows");
+    // Row, tile and chunk indexes are uint32 in the kernels: the end of the last
+    // pass tile (6144 rows at most) and of the last probe chunk (16384 rows)
+    // must stay below 2^32.
+    if (t->n_rows > VJ_MAX_ROWS)
+        return Err::make(VJ_ERR_UNSUPPORTED, "tables must have at most 2^32 - 2^16 rows");
     if (t->n_rows > 0)
         for (int c = 0; c < t->n_cols; c++)
             if (!t->cols[c]) return Err::make(VJ_ERR_INVALID_ARGUMENT, "key column pointer is NULL");

@@ -81,10 +81,13 @@ void free_all(vj_ctx *c) {
     free_buf(c->meta);
 }
 
-Err ensure(DevBuf &b, size_t bytes) {
+// Grows `b` to at least `bytes`. Without a budget an arena that must grow
+// takes 1/8 more, so slowly growing shapes do not reallocate every time; under
+// a budget it takes exactly what admit() counted.
+Err ensure(DevBuf &b, size_t bytes, bool slack) {
     if (b.bytes >= bytes) return Err::ok();
     free_buf(b);
-    size_t want = align_up(bytes + bytes / 8);
+    size_t want = align_up(slack ? bytes + bytes / 8 : bytes);
     if (hipMalloc(&b.p, want) != hipSuccess) {
         (void)hipGetLastError();
         b.p = nullptr;
@@ -132,6 +135,9 @@ struct Plan {
     // meta layout
     size_t off_jobs = 0, off_joins = 0, off_cs = 0, off_hb0 = 0, off_pt0 = 0, off_pc0 = 0, upload_bytes = 0;
     size_t off_jtasks = 0, off_task0 = 0, off_count = 0, off_ctl = 0, meta_bytes = 0;
+    // Bytes downloaded after a launch: the match counts and the control words
+    // behind them (the queue's task total).
+    size_t down_bytes() const { return off_ctl + sizeof(Ctl) - off_count; }
 };
 
 int ceil_log2(uint64_t v) {
@@ -424,38 +430,42 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
     return Err::ok();
 }
 
-uint64_t plan_scratch_bytes(const Plan &pl) {
-    return align_up(pl.tuples_bytes) + align_up(pl.counts_words * 4) + align_up(pl.base_words * 4) +
-           align_up((pl.ptask_words + pl.taskmap_cap) * 8) + align_up(pl.meta_bytes);
+// Arena sizes of a plan, as allocate() requests them (ensure() rounds each up
+// to kAlign; under a budget it adds nothing else).
+struct ArenaSizes {
+    uint64_t tuples, counts, base, ptask, meta;
+    uint64_t sum() const { return tuples + counts + base + ptask + meta; }
+};
+ArenaSizes arena_sizes(const Plan &pl) {
+    return {align_up(std::max<size_t>(pl.tuples_bytes, kAlign)), align_up(std::max<size_t>(pl.counts_words * 4, kAlign)),
+            align_up(std::max<size_t>(pl.base_words * 4, kAlign)),
+            align_up(std::max<size_t>((pl.ptask_words + pl.taskmap_cap) * 8, kAlign)), align_up(pl.meta_bytes)};
 }
+
+uint64_t plan_scratch_bytes(const Plan &pl) { return arena_sizes(pl).sum(); }
 
 // Admission (DeviceConfig::is_out_of_memory, join_provider.cuh:148-150): the
 // scratch of this pipeline plus `extra` output bytes must fit the budget.
+// Under a budget the arenas are allocated at exactly these sizes, so the
+// context then holds at most budget - extra bytes.
 Err admit(vj_ctx *c, const Plan &pl, uint64_t extra) {
     if (!c->budget) return Err::ok();
-    const uint64_t need = plan_scratch_bytes(pl) + extra;
+    const ArenaSizes a = arena_sizes(pl);
+    const uint64_t need = a.sum() + extra;
     if (need > c->budget)
         return Err::make(VJ_ERR_BUDGET, "Not enough memory for swap tables and hashes: need " + std::to_string(need) +
                                             " bytes, budget " + std::to_string(c->budget));
     // drop larger-than-needed arenas when keeping them would break the budget
-    const uint64_t keep = std::max<uint64_t>(c->tuples.bytes, pl.tuples_bytes) +
-                          std::max<uint64_t>(c->counts.bytes, pl.counts_words * 4) +
-                          std::max<uint64_t>(c->base.bytes, pl.base_words * 4) +
-                          std::max<uint64_t>(c->ptask.bytes, (pl.ptask_words + pl.taskmap_cap) * 8) +
-                          std::max<uint64_t>(c->meta.bytes, pl.meta_bytes);
+    const uint64_t keep = std::max<uint64_t>(c->tuples.bytes, a.tuples) + std::max<uint64_t>(c->counts.bytes, a.counts) +
+                          std::max<uint64_t>(c->base.bytes, a.base) + std::max<uint64_t>(c->ptask.bytes, a.ptask) +
+                          std::max<uint64_t>(c->meta.bytes, a.meta);
     if (keep + extra > c->budget) free_all(c);
     return Err::ok();
 }
 
-Err allocate(vj_ctx *c, Plan &pl) {
-    VJ_TRY(ensure(c->tuples, std::max<size_t>(pl.tuples_bytes, kAlign)));
-    VJ_TRY(ensure(c->counts, std::max<size_t>(pl.counts_words * 4, kAlign)));
-    VJ_TRY(ensure(c->base, std::max<size_t>(pl.base_words * 4, kAlign)));
-    VJ_TRY(ensure(c->ptask, std::max<size_t>((pl.ptask_words + pl.taskmap_cap) * 8, kAlign)));
-    VJ_TRY(ensure(c->meta, pl.meta_bytes));
-    VJ_TRY(ensure_pinned(c, pl.meta_bytes));
-    // tuple buffer pointers
-    char *tb = static_cast<char *>(c->tuples.p);
+// Points every partitioned job's mid / fin tuple columns into the tuple
+// arena at `tb` (offsets from the plan).
+void place_tuples(Plan &pl, char *tb) {
     for (size_t j = 0; j < pl.jobs.size(); j++) {
         JobDesc &jd = pl.jobs[j];
         const size_t col = align_up((size_t)pl.shape.kw * jd.n);
@@ -470,6 +480,17 @@ Err allocate(vj_ctx *c, Plan &pl) {
             jd.mid_ids = reinterpret_cast<uint32_t *>(p + pl.shape.nc * col);
         }
     }
+}
+
+Err allocate(vj_ctx *c, Plan &pl) {
+    const bool slack = c->budget == 0;
+    VJ_TRY(ensure(c->tuples, std::max<size_t>(pl.tuples_bytes, kAlign), slack));
+    VJ_TRY(ensure(c->counts, std::max<size_t>(pl.counts_words * 4, kAlign), slack));
+    VJ_TRY(ensure(c->base, std::max<size_t>(pl.base_words * 4, kAlign), slack));
+    VJ_TRY(ensure(c->ptask, std::max<size_t>((pl.ptask_words + pl.taskmap_cap) * 8, kAlign), slack));
+    VJ_TRY(ensure(c->meta, pl.meta_bytes, slack));
+    VJ_TRY(ensure_pinned(c, pl.meta_bytes));
+    place_tuples(pl, static_cast<char *>(c->tuples.p));
     // kept build sides: their own allocation receives the last pass and the bases
     for (auto &kp : pl.keeps) {
         vj_built *b = kp.second;
@@ -485,8 +506,20 @@ Err allocate(vj_ctx *c, Plan &pl) {
     return Err::ok();
 }
 
-PipeArgs make_args(vj_ctx *c, const Plan &pl) {
-    char *m = static_cast<char *>(c->meta.p);
+// Device and staging memory of one pipeline launch: the context's arenas, or
+// the fixed scratch of the no-allocation entry point.
+struct Arenas {
+    char *counts, *base, *ptask, *meta;  // device
+    char *pinned;                        // host staging of the descriptors and the match counts
+};
+
+Arenas ctx_arenas(vj_ctx *c) {
+    return {static_cast<char *>(c->counts.p), static_cast<char *>(c->base.p), static_cast<char *>(c->ptask.p),
+            static_cast<char *>(c->meta.p), static_cast<char *>(c->pinned)};
+}
+
+PipeArgs make_args(const Arenas &ar, const Plan &pl) {
+    char *m = ar.meta;
     PipeArgs a;
     memset(&a, 0, sizeof(a));
     a.jobs = reinterpret_cast<const JobDesc *>(m + pl.off_jobs);
@@ -499,9 +532,9 @@ PipeArgs make_args(vj_ctx *c, const Plan &pl) {
     a.task0 = reinterpret_cast<unsigned long long *>(m + pl.off_task0);
     a.out_count = reinterpret_cast<unsigned long long *>(m + pl.off_count);
     a.ctl = reinterpret_cast<Ctl *>(m + pl.off_ctl);
-    a.counts = static_cast<uint32_t *>(c->counts.p);
-    a.base = static_cast<uint32_t *>(c->base.p);
-    a.ptask = static_cast<unsigned long long *>(c->ptask.p);
+    a.counts = reinterpret_cast<uint32_t *>(ar.counts);
+    a.base = reinterpret_cast<uint32_t *>(ar.base);
+    a.ptask = reinterpret_cast<unsigned long long *>(ar.ptask);
     a.taskmap = a.ptask + pl.ptask_words;
     a.taskmap_cap = pl.taskmap_cap;
     a.max_nparts = pl.max_nparts;
@@ -518,8 +551,7 @@ PipeArgs make_args(vj_ctx *c, const Plan &pl) {
 }
 
 // Writes the descriptors into the pinned staging buffer.
-void stage_descriptors(vj_ctx *c, const Plan &pl) {
-    char *h = static_cast<char *>(c->pinned);
+void stage_descriptors(char *h, const Plan &pl) {
     memcpy(h + pl.off_jobs, pl.jobs.data(), pl.jobs.size() * sizeof(JobDesc));
     if (!pl.joins.empty()) memcpy(h + pl.off_joins, pl.joins.data(), pl.joins.size() * sizeof(JoinDesc));
     if (!pl.colscan.empty()) memcpy(h + pl.off_cs, pl.colscan.data(), pl.colscan.size() * sizeof(ColScanDesc));
@@ -528,15 +560,9 @@ void stage_descriptors(vj_ctx *c, const Plan &pl) {
     memcpy(h + pl.off_pc0, pl.p2_chunk0.data(), pl.p2_chunk0.size() * 4);
 }
 
-Err upload(vj_ctx *c, const Plan &pl, hipStream_t s) {
-    char *h = static_cast<char *>(c->pinned);
-    memcpy(h + pl.off_jobs, pl.jobs.data(), pl.jobs.size() * sizeof(JobDesc));
-    if (!pl.joins.empty()) memcpy(h + pl.off_joins, pl.joins.data(), pl.joins.size() * sizeof(JoinDesc));
-    if (!pl.colscan.empty()) memcpy(h + pl.off_cs, pl.colscan.data(), pl.colscan.size() * sizeof(ColScanDesc));
-    memcpy(h + pl.off_hb0, pl.hist_block0.data(), pl.hist_block0.size() * 4);
-    memcpy(h + pl.off_pt0, pl.p1_tile0.data(), pl.p1_tile0.size() * 4);
-    memcpy(h + pl.off_pc0, pl.p2_chunk0.data(), pl.p2_chunk0.size() * 4);
-    VJ_HIP_TRY(hipMemcpyAsync(c->meta.p, h, pl.upload_bytes, hipMemcpyHostToDevice, s));
+Err upload(const Arenas &ar, const Plan &pl, hipStream_t s) {
+    stage_descriptors(ar.pinned, pl);
+    VJ_HIP_TRY(hipMemcpyAsync(ar.meta, ar.pinned, pl.upload_bytes, hipMemcpyHostToDevice, s));
     return Err::ok();
 }
 
@@ -606,7 +632,7 @@ Err capture_pipeline(vj_ctx *c, const Plan &pl, const PipeArgs &a) {
     Err e = he == hipSuccess ? launch_stages(c, pl, a, cs, true, false)
                              : Err::make(VJ_ERR_HIP, std::string("capture upload: ") + hipGetErrorString(he));
     if (!e.failed()) {
-        he = hipMemcpyAsync(h + pl.off_count, static_cast<char *>(c->meta.p) + pl.off_count, pl.joins.size() * 8,
+        he = hipMemcpyAsync(h + pl.off_count, static_cast<char *>(c->meta.p) + pl.off_count, pl.down_bytes(),
                             hipMemcpyDeviceToHost, cs);
         if (he != hipSuccess) e = Err::make(VJ_ERR_HIP, std::string("capture download: ") + hipGetErrorString(he));
     }
@@ -635,7 +661,31 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-void fill_summary(vj_ctx *c, const Plan &pl, const JoinSpec &js, uint64_t matches, int retries, bool replay) {
+// Algorithmic bytes of each stage of a one-join plan: the histogram reads the
+// keys of every side it partitions; pass 1 reads keys (and explicit row ids)
+// and writes keys + row ids; pass 2 reads and writes keys + row ids; build +
+// probe reads every tuple once and writes 8 bytes per pair. Resident build
+// sides were partitioned earlier and only count in build + probe.
+void stage_bytes(const Plan &pl, uint64_t matches, vj_join_summary &s) {
+    const uint64_t kb = (uint64_t)pl.kb;
+    for (const JobDesc &jd : pl.jobs) {
+        const uint64_t n = jd.n, ids_in = jd.in.rowids ? 4 : 0;
+        const bool resident = jd.res_base != nullptr;
+        if (jd.b1 && !resident) {
+            s.bytes_histogram += n * kb;
+            s.bytes_pass1 += n * (kb + ids_in) + n * (kb + 4);
+        }
+        if (jd.b2 && !resident) s.bytes_pass2 += 2 * n * (kb + 4);
+        s.bytes_build_probe += jd.b1 || resident ? n * (kb + 4) : n * (kb + ids_in);
+    }
+    s.bytes_build_probe += matches * 8;
+    s.bytes_total = s.bytes_histogram + s.bytes_pass1 + s.bytes_pass2 + s.bytes_build_probe;
+}
+
+double gbps(uint64_t bytes, double ms) { return ms > 0 ? (double)bytes / (ms * 1e-3) / 1e9 : 0.0; }
+
+void fill_summary(vj_ctx *c, const Plan &pl, const JoinSpec &js, uint64_t matches, uint64_t tasks, int retries,
+                  bool replay, uint64_t scratch_bytes) {
     vj_join_summary &s = c->summary;
     memset(&s, 0, sizeof(s));
     s.build_rows = js.res ? js.res->n : js.r->n_rows;
@@ -647,6 +697,7 @@ void fill_summary(vj_ctx *c, const Plan &pl, const JoinSpec &js, uint64_t matche
     s.bits2 = jr.b2;
     s.retries = retries;
     s.partitions = pl.joins[0].nparts;
+    s.tasks = tasks;
     s.ms_total = elapsed(c->ev[EV_START], c->ev[EV_PROBE]);
     if (c->profiling) {
         s.ms_histogram = elapsed(c->ev[EV_START], c->ev[EV_HIST]);
@@ -654,8 +705,15 @@ void fill_summary(vj_ctx *c, const Plan &pl, const JoinSpec &js, uint64_t matche
         s.ms_pass2 = elapsed(c->ev[EV_P1], c->ev[EV_P2]);
         s.ms_build_probe = elapsed(c->ev[EV_P2], c->ev[EV_PROBE]);
     }
-    s.scratch_bytes = held_bytes(c);
+    s.scratch_bytes = scratch_bytes;
     s.graph_replay = replay ? 1 : 0;
+    stage_bytes(pl, matches, s);
+    s.gbps_histogram = gbps(s.bytes_histogram, s.ms_histogram);
+    s.gbps_pass1 = gbps(s.bytes_pass1, s.ms_pass1);
+    s.gbps_pass2 = gbps(s.bytes_pass2, s.ms_pass2);
+    s.gbps_build_probe = gbps(s.bytes_build_probe, s.ms_build_probe);
+    s.gbps_total = gbps(s.bytes_total, s.ms_total);
+    s.probe_tuples_per_s = s.ms_total > 0 ? (double)s.probe_rows / (s.ms_total * 1e-3) : 0.0;
 }
 
 vj_status fail(vj_ctx *c, const Err &e) {
@@ -682,7 +740,7 @@ Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hi
     VJ_TRY(build_join_plan(pl, specs, o, c));
     VJ_TRY(admit(c, pl, extra_bytes + pl.keep_bytes));
     VJ_TRY(allocate(c, pl));
-    PipeArgs a = make_args(c, pl);
+    PipeArgs a = make_args(ctx_arenas(c), pl);
     char *h = static_cast<char *>(c->pinned);
     unsigned long long *hc = reinterpret_cast<unsigned long long *>(h + pl.off_count);
     bool replay = false;
@@ -694,7 +752,7 @@ Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hi
             replay = true;
         } else if (gmode == GRAPH_ON || sig == c->last_sig) {
             // the previous use of the pinned buffer has completed (every call ends synchronised)
-            stage_descriptors(c, pl);
+            stage_descriptors(static_cast<char *>(c->pinned), pl);
             const Err ce = capture_pipeline(c, pl, a);
             if (ce.failed()) {
                 if (gmode == GRAPH_ON) return ce;
@@ -708,21 +766,28 @@ Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hi
         }
     }
     if (replay) {
-        stage_descriptors(c, pl);  // the graph's upload node reads them at launch
+        stage_descriptors(static_cast<char *>(c->pinned), pl);  // the graph's upload node reads them at launch
         VJ_HIP_TRY(hipEventRecord(c->ev[EV_START], s));
         VJ_HIP_TRY(hipGraphLaunch(c->graph, s));
         VJ_HIP_TRY(hipEventRecord(c->ev[EV_PROBE], s));
         c->graph_replays++;
     } else {
-        VJ_TRY(upload(c, pl, s));
+        VJ_TRY(upload(ctx_arenas(c), pl, s));
         VJ_TRY(launch_stages(c, pl, a, s, true));
-        VJ_HIP_TRY(hipMemcpyAsync(hc, static_cast<char *>(c->meta.p) + pl.off_count, pl.joins.size() * 8,
+        VJ_HIP_TRY(hipMemcpyAsync(hc, static_cast<char *>(c->meta.p) + pl.off_count, pl.down_bytes(),
                                   hipMemcpyDeviceToHost, s));
     }
     VJ_HIP_TRY(hipStreamSynchronize(s));
     counts.assign(hc, hc + pl.joins.size());
     int retries = 0;
     if (grow) {
+        // the regrown outputs are admitted like the first ones: the context's
+        // arenas plus every output it allocates must fit the budget
+        uint64_t out_bytes = 0;
+        for (size_t i = 0; i < specs.size(); i++) out_bytes += std::max<uint64_t>(counts[i], specs[i].cap) * 8;
+        if (c->budget && held_bytes(c) + out_bytes > c->budget)
+            return Err::make(VJ_ERR_BUDGET, "the join has " + std::to_string(counts[0]) +
+                                                " pairs; its output does not fit the HBM budget");
         bool again = false;
         for (size_t i = 0; i < specs.size(); i++) {
             if (counts[i] > specs[i].cap) {
@@ -749,14 +814,16 @@ Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hi
             VJ_LAUNCH(launch_reset_probe(a, s));
             VJ_LAUNCH(launch_probe(a, pl.shape, c->cus * probe_blocks_per_cu(pl.shape), s));
             VJ_HIP_TRY(hipEventRecord(c->ev[EV_PROBE], s));
-            VJ_HIP_TRY(hipMemcpyAsync(hc, static_cast<char *>(c->meta.p) + pl.off_count, pl.joins.size() * 8,
+            VJ_HIP_TRY(hipMemcpyAsync(hc, static_cast<char *>(c->meta.p) + pl.off_count, pl.down_bytes(),
                                       hipMemcpyDeviceToHost, s));
             VJ_HIP_TRY(hipStreamSynchronize(s));
             counts.assign(hc, hc + pl.joins.size());
         }
     }
     if (retries_out) *retries_out = retries;
-    if (specs.size() == 1) fill_summary(c, pl, specs[0], counts[0], retries, replay);
+    if (specs.size() == 1)
+        fill_summary(c, pl, specs[0], counts[0], reinterpret_cast<const Ctl *>(h + pl.off_ctl)->task_total, retries,
+                     replay, held_bytes(c));
     return Err::ok();
 }
 
@@ -767,8 +834,9 @@ namespace vj {
 Err pipeline_bytes(const std::vector<JoinSpec> &specs, const vj_join_opts *o, uint64_t *bytes) {
     Plan pl;
     VJ_TRY(build_join_plan(pl, specs, o));
-    const uint64_t scratch = plan_scratch_bytes(pl);
-    *bytes = scratch + scratch / 8 + pl.keep_bytes;
+    // a lane launches under a budget, so its arenas are allocated at exactly
+    // these sizes (no growth slack)
+    *bytes = plan_scratch_bytes(pl) + pl.keep_bytes;
     return Err::ok();
 }
 
@@ -780,11 +848,11 @@ Err pipeline_launch(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts 
     c->budget = budget;
     VJ_TRY(admit(c, pl, pl.keep_bytes));
     VJ_TRY(allocate(c, pl));
-    VJ_TRY(upload(c, pl, s));
-    PipeArgs a = make_args(c, pl);
+    VJ_TRY(upload(ctx_arenas(c), pl, s));
+    PipeArgs a = make_args(ctx_arenas(c), pl);
     VJ_TRY(launch_stages(c, pl, a, s, true));
     char *h = static_cast<char *>(c->pinned);
-    VJ_HIP_TRY(hipMemcpyAsync(h + pl.off_count, static_cast<char *>(c->meta.p) + pl.off_count, pl.joins.size() * 8,
+    VJ_HIP_TRY(hipMemcpyAsync(h + pl.off_count, static_cast<char *>(c->meta.p) + pl.off_count, pl.down_bytes(),
                               hipMemcpyDeviceToHost, s));
     *count_off = pl.off_count;
     return Err::ok();
@@ -915,6 +983,8 @@ vj_status vj_join(vj_ctx *c, const vj_table *build, const vj_table *probe, const
     e = set_device(c);
     if (e.failed()) return fail(c, e);
     uint64_t cap = opts->out_capacity_hint ? opts->out_capacity_hint : std::max<uint64_t>(probe->n_rows, 1);
+    if (c->budget && cap * 8 > c->budget)  // the output alone breaks the budget: nothing is allocated
+        return fail(c, Err::make(VJ_ERR_BUDGET, "an output of " + std::to_string(cap) + " pairs exceeds the HBM budget"));
     std::vector<JoinSpec> specs{{build, probe, nullptr, nullptr, cap}};
     if (hipMalloc(&specs[0].out_r, cap * 4) != hipSuccess || hipMalloc(&specs[0].out_s, cap * 4) != hipSuccess) {
         (void)hipGetLastError();
@@ -982,9 +1052,9 @@ vj_status vj_partition(vj_ctx *c, const vj_table *table, const vj_hash_opts *has
     JobDesc &jd = pl.jobs[0];
     for (int cc = 0; cc < 4; cc++) jd.fin_cols[cc] = cc < out->n_cols ? const_cast<void *>(out->cols[cc]) : nullptr;
     jd.fin_ids = const_cast<uint32_t *>(out->rowids);
-    e = upload(c, pl, s);
+    e = upload(ctx_arenas(c), pl, s);
     if (e.failed()) return fail(c, e);
-    PipeArgs a = make_args(c, pl);
+    PipeArgs a = make_args(ctx_arenas(c), pl);
     e = launch_stages(c, pl, a, s, false);
     if (e.failed()) return fail(c, e);
     hipError_t he = launch_emit_hist(a.base + jd.base_off, 1u << bits, hist, offsets, s);
@@ -1066,9 +1136,9 @@ vj_status vj_build(vj_ctx *c, const vj_table *build, const vj_join_opts *opts, v
         e = admit(c, pl, pl.keep_bytes);
     }
     if (!e.failed()) e = allocate(c, pl);
-    if (!e.failed()) e = upload(c, pl, s);
+    if (!e.failed()) e = upload(ctx_arenas(c), pl, s);
     if (!e.failed()) {
-        PipeArgs a = make_args(c, pl);
+        PipeArgs a = make_args(ctx_arenas(c), pl);
         e = launch_stages(c, pl, a, s, false);
     }
     if (!e.failed()) {

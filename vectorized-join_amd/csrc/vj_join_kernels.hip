@@ -938,9 +938,9 @@ __device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint16_t *
     }
 }
 
-// Waves per SIMD the probe kernel is compiled for: 6 (80 VGPRs, three blocks
+// Waves per SIMD the probe kernel is compiled for: 8 (64 VGPRs, four blocks
 // per CU) for single 8-byte keys with FNV / XOR hashing; wider keys and the
-// generic hash path get 4 (128 VGPRs) so they do not spill.
+// generic hash path get 4 (128 VGPRs, two blocks per CU) so they do not spill.
 template <int HP, int KB> constexpr int probe_waves_per_simd() { return KB <= 8 && HP != HP_GENERIC ? 8 : 4; }
 
 template <int HP, int KW, int NC>

@@ -215,6 +215,10 @@ LaunchResult launch_batch(vj_server *sv, Lane &L, Lane &O, std::unique_lock<std:
         hipError_t he = hipEventRecord(L.ev, L.s);
         if (he != hipSuccess) e = Err::make(VJ_ERR_HIP, hipGetErrorString(he));
     }
+    // A launch that failed part-way may have queued stages that still read the
+    // requests' tables and write their outputs: they finish before the
+    // requests are reported failed (and their buffers can be freed).
+    if (e.failed()) (void)hipStreamSynchronize(L.s);
     lk.lock();
     for (Req *r : batch)
         if (r->kept) L.kept_bytes += r->kept->bytes;

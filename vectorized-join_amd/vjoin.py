@@ -76,7 +76,13 @@ class JoinSummary(ctypes.Structure):
                 ("retries", ctypes.c_int32), ("partitions", ctypes.c_uint64), ("tasks", ctypes.c_uint64),
                 ("ms_total", ctypes.c_double), ("ms_histogram", ctypes.c_double), ("ms_pass1", ctypes.c_double),
                 ("ms_pass2", ctypes.c_double), ("ms_build_probe", ctypes.c_double),
-                ("scratch_bytes", ctypes.c_uint64), ("graph_replay", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("scratch_bytes", ctypes.c_uint64), ("graph_replay", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("bytes_histogram", ctypes.c_uint64), ("bytes_pass1", ctypes.c_uint64),
+                ("bytes_pass2", ctypes.c_uint64), ("bytes_build_probe", ctypes.c_uint64),
+                ("bytes_total", ctypes.c_uint64), ("gbps_histogram", ctypes.c_double),
+                ("gbps_pass1", ctypes.c_double), ("gbps_pass2", ctypes.c_double),
+                ("gbps_build_probe", ctypes.c_double), ("gbps_total", ctypes.c_double),
+                ("probe_tuples_per_s", ctypes.c_double)]
 
 
 def lib():

@@ -182,6 +182,29 @@ vj_status vj_join(vj_ctx *ctx, const vj_table *build, const vj_table *probe, con
 vj_status vj_join_into(vj_ctx *ctx, const vj_table *build, const vj_table *probe, const vj_join_opts *opts,
                        uint32_t *out_build, uint32_t *out_probe, uint64_t capacity, uint64_t *count, void *stream);
 
+/*
+ * No-allocation variant of vj_join_into. It has no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It calls no hipMalloc, hipFree, hipHostMalloc or other allocating routine:
+ *  - the pipeline's scratch is a fixed array of VJ_NOALLOC_SCRATCH_BYTES per
+ *    device, which the library holds on every GPU it runs on until the
+ *    process ends; its address is looked up once per device by vj_ctx_create;
+ *  - descriptors and match counts pass through a small pinned block the
+ *    context allocated in vj_ctx_create.
+ * A join whose scratch would exceed the array fails with VJ_ERR_UNSUPPORTED
+ * before anything is launched. Two radix passes need about
+ * (build rows + probe rows) x (key bytes + 4) x 2 bytes, one pass half that.
+ * Every call on one GPU uses that GPU's array, whichever context it comes
+ * from: calls that overlap on one GPU race (run them in order). Nothing is
+ * captured into or replayed from a hipGraph. Results, capacity handling and
+ * the summary are those of vj_join_into.
+ */
+#define VJ_NOALLOC_SCRATCH_BYTES (64ull << 20)
+vj_status vj_join_into_noalloc(vj_ctx *ctx, const vj_table *build, const vj_table *probe, const vj_join_opts *opts,
+                               uint32_t *out_build, uint32_t *out_probe, uint64_t capacity, uint64_t *count,
+                               void *stream);
+
 vj_status vj_pairs_free(vj_pairs *pairs);
 
 /* Per-join statistics of the last join on the context (JoinSummary,

@@ -23,6 +23,7 @@ using namespace vj;
 namespace {
 
 constexpr size_t kAlign = 256;
+constexpr size_t kNoAllocStageBytes = 64 << 10;  // descriptors + counts of one join (a few KiB)
 
 inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
@@ -57,6 +58,12 @@ struct vj_ctx {
     uint64_t graph_replays = 0, graph_captures = 0;
     // Autotuned radix plans (vj_tune): size class -> passes | bits1 << 8 | bits2 << 16
     std::unordered_map<uint64_t, uint32_t> tuned;
+    // vj_join_into_noalloc: this device's fixed scratch (looked up at creation)
+    // and a pinned staging block for its descriptors and counts (allocated at
+    // creation, so the entry point itself allocates nothing)
+    void *na_scratch = nullptr;
+    void *na_stage = nullptr;
+    std::string na_error;
 };
 
 namespace {
@@ -898,6 +905,15 @@ vj_status vj_ctx_create(int device, vj_ctx **out) {
             return VJ_ERR_HIP;
         }
     }
+    // set-up of the no-allocation entry point (its failure only disables that entry point)
+    const Err ne = noalloc_scratch(device, &c->na_scratch);
+    if (ne.failed()) {
+        c->na_error = ne.msg;
+    } else if (hipHostMalloc(&c->na_stage, kNoAllocStageBytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        c->na_stage = nullptr;
+        c->na_error = "hipHostMalloc of the staging block failed";
+    }
     *out = c;
     return VJ_OK;
 }
@@ -910,6 +926,7 @@ vj_status vj_ctx_destroy(vj_ctx *c) {
     if (c->capture_stream) (void)hipStreamDestroy(c->capture_stream);
     free_all(c);
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->na_stage) (void)hipHostFree(c->na_stage);
     for (int i = 0; i < EV_COUNT; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     delete c;
@@ -967,6 +984,64 @@ vj_status vj_join_into(vj_ctx *c, const vj_table *build, const vj_table *probe, 
     if (counts[0] > capacity)
         return fail(c, Err::make(VJ_ERR_CAPACITY, "output holds " + std::to_string(capacity) + " pairs, join has " +
                                                       std::to_string(counts[0])));
+    return VJ_OK;
+}
+
+vj_status vj_join_into_noalloc(vj_ctx *c, const vj_table *build, const vj_table *probe, const vj_join_opts *opts,
+                               uint32_t *out_build, uint32_t *out_probe, uint64_t capacity, uint64_t *count,
+                               void *stream) {
+    if (!c || !count) return VJ_ERR_INVALID_ARGUMENT;
+    c->last_error.clear();
+    *count = 0;
+    vj_join_opts d;
+    vj_join_opts_default(&d);
+    if (!opts) opts = &d;
+    if (capacity && (!out_build || !out_probe)) return fail(c, Err::make(VJ_ERR_INVALID_ARGUMENT, "output is NULL"));
+    Err e = check_pair(build, probe);
+    if (e.failed()) return fail(c, e);
+    if (!c->na_scratch || !c->na_stage)
+        return fail(c, Err::make(VJ_ERR_UNSUPPORTED, "no-allocation scratch unavailable: " + c->na_error));
+    e = set_device(c);
+    if (e.failed()) return fail(c, e);
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<JoinSpec> specs{{build, probe, out_build, out_probe, capacity}};
+    Plan pl;
+    e = build_join_plan(pl, specs, opts, c);
+    if (e.failed()) return fail(c, e);
+    // the arenas, laid out back to back in the fixed scratch
+    const ArenaSizes az = arena_sizes(pl);
+    if (az.sum() > kNoAllocScratchBytes)
+        return fail(c, Err::make(VJ_ERR_UNSUPPORTED, "this join needs " + std::to_string(az.sum()) +
+                                                         " bytes of scratch; the no-allocation entry point has " +
+                                                         std::to_string(kNoAllocScratchBytes) +
+                                                         " (vj_join_into has no such limit)"));
+    if (pl.meta_bytes > kNoAllocStageBytes)
+        return fail(c, Err::make(VJ_ERR_UNSUPPORTED, "this join's descriptors exceed the no-allocation staging block"));
+    char *base = static_cast<char *>(c->na_scratch);
+    Arenas ar;
+    ar.counts = base + az.tuples;
+    ar.base = ar.counts + az.counts;
+    ar.ptask = ar.base + az.base;
+    ar.meta = ar.ptask + az.ptask;
+    ar.pinned = static_cast<char *>(c->na_stage);
+    place_tuples(pl, base);
+    const PipeArgs a = make_args(ar, pl);
+    e = upload(ar, pl, s);
+    if (!e.failed()) e = launch_stages(c, pl, a, s, true);
+    if (!e.failed()) {
+        hipError_t he = hipMemcpyAsync(ar.pinned + pl.off_count, ar.meta + pl.off_count, pl.down_bytes(),
+                                       hipMemcpyDeviceToHost, s);
+        if (he == hipSuccess) he = hipStreamSynchronize(s);
+        if (he != hipSuccess) e = Err::make(VJ_ERR_HIP, std::string("no-allocation join: ") + hipGetErrorString(he));
+    }
+    if (e.failed()) return fail(c, e);
+    const uint64_t m = *reinterpret_cast<const unsigned long long *>(ar.pinned + pl.off_count);
+    fill_summary(c, pl, specs[0], m, reinterpret_cast<const Ctl *>(ar.pinned + pl.off_ctl)->task_total, 0, false,
+                 az.sum());
+    *count = m;
+    if (m > capacity)
+        return fail(c, Err::make(VJ_ERR_CAPACITY, "output holds " + std::to_string(capacity) + " pairs, join has " +
+                                                      std::to_string(m)));
     return VJ_OK;
 }
 

@@ -47,4 +47,12 @@ Err launch_hash(const vj_table *t, const HashCfg &cfg, void *out, hipStream_t s,
 
 int device_cu_count(int device);
 
+// vj_noalloc.hip: the fixed device scratch of vj_join_into_noalloc.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;  // per device, held until the process ends
+static_assert(kNoAllocScratchBytes == VJ_NOALLOC_SCRATCH_BYTES, "the header states the scratch size");
+constexpr int kNoAllocMaxDevices = 64;
+// Device address of `device`'s copy of the scratch array; the first call for a
+// device (made with that device current) looks it up, later calls return it.
+Err noalloc_scratch(int device, void **ptr);
+
 }  // namespace vj

@@ -114,6 +114,7 @@ def lib():
             "vj_join_opts_default": ([P(JoinOpts)], None),
             "vj_join": ([vp, P(Table), P(Table), P(JoinOpts), P(Pairs), vp], i32),
             "vj_join_into": ([vp, P(Table), P(Table), P(JoinOpts), vp, vp, u64, P(u64), vp], i32),
+            "vj_join_into_noalloc": ([vp, P(Table), P(Table), P(JoinOpts), vp, vp, u64, P(u64), vp], i32),
             "vj_pairs_free": ([P(Pairs)], i32),
             "vj_join_summary_get": ([vp, P(JoinSummary)], i32),
             "vj_partition": ([vp, P(Table), P(HashOpts), i32, i32, P(Table), vp, vp, vp], i32),
@@ -244,6 +245,15 @@ class Context:
                                 ctypes.c_void_p(out_r.data_ptr() if out_r is not None else 0),
                                 ctypes.c_void_p(out_s.data_ptr() if out_s is not None else 0), cap,
                                 ctypes.byref(cnt), stream_ptr())
+        return rc, cnt.value
+
+    def join_into_noalloc(self, build, probe, opts, out_r, out_s, capacity=None):
+        """vj_join_into_noalloc (fixed per-device scratch, nothing allocated); returns (status, count)."""
+        cnt = ctypes.c_uint64()
+        cap = int(out_r.numel()) if capacity is None else int(capacity)
+        rc = lib().vj_join_into_noalloc(self.ptr, ctypes.byref(build), ctypes.byref(probe), ctypes.byref(opts),
+                                        ctypes.c_void_p(out_r.data_ptr()), ctypes.c_void_p(out_s.data_ptr()), cap,
+                                        ctypes.byref(cnt), stream_ptr())
         return rc, cnt.value
 
     def join(self, build, probe, opts):

@@ -3,17 +3,21 @@
 
 A step is one complete join of the large FK workload: build 2^24 unique int64
 keys, probe 2^27 int64 keys drawn from the build keys (100 % match), FNV-1a-64,
-2 radix passes (bits chosen by the planner), all (build_rowid, probe_rowid)
-pairs written to HBM. Inputs are resident in HBM before the timed region; the
-output buffers are allocated before it too (allocation is timed separately).
+2 radix passes (bits autotuned), all (build_rowid, probe_rowid) pairs written
+to HBM. Inputs are resident in HBM before the timed region; the output buffers
+are allocated before it too (allocation is timed separately).
 
 Prints one JSON line. `value` is probe tuples per second of the whole join.
 `roofline` is the dominant kernel's algorithmic bytes per launch over its
 average duration, from HIP events the library records on the join's stream
-around each stage. `cpu_baseline` is the oracle's OpenMP implementation of the
-same algorithm on the host cores, on a bounded sample of the workload.
+around each stage. Further legs, in the same line:
+  multicol_skew  the multi-column Zipf join (2^22 x 2^25, 4 x uint32 keys)
+  hash           the standalone hash kernel over HASH_BITS x chunk widths
+  batched        256 uneven joins, one batched pipeline vs. sequential launches
+  serving        a Poisson request stream, served vs. one join per launch
+  cpu_baseline   the same algorithm (the OpenMP oracle) on the host cores
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--log-r 24 --log-s 27]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs fk,multicol,hash,batched,serving,cpu]
 """
 import argparse
 import json
@@ -48,25 +52,56 @@ def stage_bytes(nr, ns, m):
     }
 
 
-def cpu_baseline(log_r, log_s, passes, bits1, bits2):
-    import numpy as np
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def cpu_baseline(r_keys, s_keys, passes, bits1, bits2, runs=5, one_thread_log_shrink=3):
+    """The same algorithm (oracle/vj_oracle.c: word-wise FNV-1a-64, LSB radix
+    digits with the GPU run's passes and bits, per-partition hash tables) on the
+    host cores: OpenMP on all of this process's cores with OMP_PROC_BIND=close,
+    the median of `runs` timed joins after one warm-up join, on the full large
+    FK join (the same seeded keys as the GPU steps; generation not timed); plus
+    a 1-thread figure on a 1/2^shrink sample of it (median of 3 after a warm-up)."""
+    import numpy as np
+    os.environ.setdefault("OMP_PROC_BIND", "close")  # read when the oracle's OpenMP runtime starts
     import oracle
-    import workloads
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    rcols, scols = workloads.large_fk(log_r=log_r, log_s=log_s)
-    r = rcols[0].view(np.uint64).reshape(-1, 1)
-    s = scols[0].view(np.uint64).reshape(-1, 1)
-    cap = len(s)
-    oracle.join(r[:4096], s[:4096], passes=1, bits1=4, nthreads=threads, cap=cap)  # warm-up
-    t0 = time.perf_counter()
-    rr, ss = oracle.join(r, s, "fnv", 64, 64, passes=passes, bits1=bits1, bits2=bits2, load=0.25, nthreads=threads,
-                         cap=cap)
-    dt = time.perf_counter() - t0
-    assert len(rr) == len(s)
-    return {"value": len(s) / dt, "unit": "probe tuples/s", "cores": threads, "kind": "port",
-            "sample": "large_fk at build 2^%d, probe 2^%d (FNV-1a-64, %d passes, bits %d+%d, OpenMP oracle "
-                      "oracle/vj_oracle.c, %.2f s)" % (log_r, log_s, passes, bits1, bits2, dt)}
+    r = r_keys.view(np.uint64).reshape(-1, 1)
+    s = s_keys.view(np.uint64).reshape(-1, 1)
+
+    def timed(rr, ss, nthreads, n_runs):
+        ts = []
+        for k in range(n_runs + 1):  # the first run is the warm-up
+            t0 = time.perf_counter()
+            out_r, _ = oracle.join(rr, ss, "fnv", 64, 64, passes=passes, bits1=bits1, bits2=bits2, load=0.25,
+                                   nthreads=nthreads, cap=len(ss))
+            dt = time.perf_counter() - t0
+            assert len(out_r) == len(ss)  # FK join: every probe row matches once
+            if k:
+                ts.append(dt)
+        return float(np.median(ts))
+
+    t_all = timed(r, s, threads, runs)
+    shrink = 1 << one_thread_log_shrink
+    r1 = r[: len(r) // shrink]
+    s1 = s[: len(s) // shrink]
+    s1 = r1[np.random.default_rng(1).integers(0, len(r1), size=len(s1))]  # FK keys of the smaller build side
+    t_one = timed(r1, s1, 1, 3)
+    return {"value": len(s) / t_all, "unit": "probe tuples/s", "cores": threads, "kind": "port",
+            "sample": "full large FK join (build %d, probe %d int64 keys), FNV-1a-64, %d passes, bits %d+%d, "
+                      "OpenMP oracle oracle/vj_oracle.c, median of %d runs after a warm-up: %.3f s" % (
+                          len(r), len(s), passes, bits1, bits2, runs, t_all),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+            "one_thread": {"value": len(s1) / t_one, "unit": "probe tuples/s", "cores": 1,
+                           "sample": "build %d, probe %d (1/%d of the join), median of 3 after a warm-up: %.3f s" % (
+                               len(r1), len(s1), shrink, t_one)}}
 
 
 # Stage -> kernel (the HIP events around a stage bracket exactly this kernel, except
@@ -140,6 +175,28 @@ def traffic_from_pmc(fetch_csv, write_csv, workload):
         wb = float(np.median(per[(st, "WRITE_SIZE")])) * factors["WRITE_SIZE"]
         lines.append("stage=%s kernel=%s fetch_bytes=%.0f write_bytes=%.0f traffic_bytes=%.0f dispatches=%d" %
                      (st, STAGE_KERNEL[st], fb, wb, fb + wb, len(per[(st, "FETCH_SIZE")])))
+    return "\n".join(lines) + "\n"
+
+
+def trace_summary(db_path, command, workload):
+    """Markdown summary of a rocprofv3 --kernel-trace database: per kernel, the
+    dispatch count, average / min / max duration and its resources."""
+    import sqlite3
+    db = sqlite3.connect(db_path)
+    rows = db.execute("select name, count(*), avg(duration), min(duration), max(duration), sum(duration), "
+                      "max(vgpr_count), max(sgpr_count), max(lds_size), max(scratch_size), max(grid_x), "
+                      "max(workgroup_x) from kernels group by name order by sum(duration) desc").fetchall()
+    db.close()
+    lines = ["# Kernel trace: " + workload, "", "Made by:", "", "```", command, "python3 bench.py --trace-summary "
+             "<that run's run_results.db> OUT.md --trace-command '...' --trace-workload '...'", "```", "",
+             "| kernel | calls | avg us | min us | max us | total us | VGPRs | SGPRs | LDS B | scratch B/lane | "
+             "grid | block |", "|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    for name, n, avg, mn, mx, tot, vg, sg, lds, scr, gx, wx in rows:
+        short = name.replace("(anonymous namespace)::", "")
+        if len(short) > 70:
+            short = short[:67] + "..."
+        lines.append("| `%s` | %d | %.1f | %.1f | %.1f | %.1f | %d | %d | %d | %d | %d | %d |" % (
+            short, n, avg / 1e3, mn / 1e3, mx / 1e3, tot / 1e3, vg, sg, lds, scr, gx, wx))
     return "\n".join(lines) + "\n"
 
 
@@ -396,23 +453,29 @@ def serving_leg(n_requests, rates, budget_gb=64, n_pool=64, n_tables=8, n_reprob
     return results
 
 
-def multicol_leg(ctx, steps):
+def multicol_data():
+    """Device tables of the multi-column skewed join and its number of hitting probe rows."""
+    import numpy as np
+
+    import workloads
+    rcols, scols = workloads.multicol_skew()
+    n_hit = int(np.count_nonzero(scols[0] < np.uint32(0x80000000)))
+    return workloads.to_device(rcols), workloads.to_device(scols), n_hit
+
+
+def multicol_leg(ctx, steps, data):
     """Multi-column skewed join: 4 x uint32 key columns, build 2^22 rows (2^20
     distinct keys, each 4 times), probe 2^25 rows (80 % Zipf(1.0) over the
     distinct keys, 20 % outside them), with FNV-1a-64 and custom_xor_hw-64.
     Checked by size-independent properties: the match count is 4 x the hitting
     probe rows, every pair has equal keys in all 4 columns, every probe row
-    appears 0 or 4 times."""
+    appears 0 or 4 times. (The tests compare the same join with the oracle.)"""
     import numpy as np
     import torch
 
     import vjoin
-    import workloads
-    rcols, scols = workloads.multicol_skew()
-    nr, ns = len(rcols[0]), len(scols[0])
-    n_hit = int(np.count_nonzero(scols[0] < np.uint32(0x80000000)))
-    dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
-    del rcols, scols
+    dr, ds, n_hit = data
+    nr, ns = int(dr[0].numel()), int(ds[0].numel())
     m_want = 4 * n_hit
     out_r = torch.empty(m_want, dtype=torch.int32, device="cuda")
     out_s = torch.empty(m_want, dtype=torch.int32, device="cuda")
@@ -444,6 +507,47 @@ def multicol_leg(ctx, steps):
     return res
 
 
+def hash_leg(shapes, reps=10):
+    """vj_hash (the standalone hash kernel, k_hash) over the probe-side keys of the
+    large FK join and of the multi-column join, for FNV-1a and custom_xor_hw,
+    HASH_BITS 32 / 64 and chunk widths 32 / 64 / 128 bits (where the row is a
+    whole number of chunks), as the reference's hash benchmark sweeps
+    (src/playground/hash_benchmark.cu:59-169). Bytes per launch:
+    B_hash = n x (key bytes read) + n x (hash bytes written); time from HIP
+    events on the launch stream, median of `reps` after a warm-up."""
+    import numpy as np
+    import torch
+
+    import vjoin
+    rows = []
+    for name, cols in shapes:
+        n = int(cols[0].numel())
+        key_bytes = sum(c.element_size() for c in cols)
+        words = 2 * len(cols)  # reference words: every key column widened to 64 bits
+        for kind in ("fnv", "custom_xor_hw"):
+            for hb in (32, 64):
+                out = torch.empty(n, dtype=torch.int32 if hb == 32 else torch.int64, device="cuda")
+                for cb in (32, 64, 128):
+                    if (words * 32) % cb:
+                        continue
+                    vjoin.hash_table(cols, kind, hb, cb, out=out)  # warm-up
+                    ts = []
+                    for _ in range(reps):
+                        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                        a.record()
+                        vjoin.hash_table(cols, kind, hb, cb, out=out)
+                        b.record()
+                        b.synchronize()
+                        ts.append(a.elapsed_time(b))
+                    ms = float(np.median(ts))
+                    B = n * key_bytes + n * hb // 8
+                    rows.append({"keys": name, "hash": kind, "hash_bits": hb, "chunk_bits": cb, "rows": n,
+                                 "bytes": B, "ms": ms, "GBps": B / (ms * 1e-3) / 1e9,
+                                 "frac": B / (ms * 1e-3) / HBM_PEAK, "tuples_per_s": n / (ms * 1e-3)})
+                del out
+    return {"kernel": "k_hash", "unit": "GB/s", "rows": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -452,9 +556,10 @@ def main():
     ap.add_argument("--log-r", type=int, default=24)
     ap.add_argument("--log-s", type=int, default=27)
     ap.add_argument("--passes", type=int, default=2)
+    ap.add_argument("--legs", default="fk,multicol,hash,batched,serving,cpu",
+                    help="comma list of legs to run: fk (the headline line), multicol, hash, batched, serving, cpu")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-log-r", type=int, default=23)
-    ap.add_argument("--cpu-log-s", type=int, default=26)
+    ap.add_argument("--no-hash", action="store_true", help="skip the hash kernel leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--batched-joins", type=int, default=256)
     ap.add_argument("--batched-reps", type=int, default=3)
@@ -468,7 +573,17 @@ def main():
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="no GPU: write the traffic summary from two rocprofv3 --pmc results (run_results.db or "
                          "counter_collection.csv)")
+    ap.add_argument("--trace-summary", nargs=2, metavar=("RUN_RESULTS_DB", "OUT_MD"),
+                    help="no GPU: write a markdown per-kernel summary of a rocprofv3 --kernel-trace database")
+    ap.add_argument("--trace-command", default="", help="the command that made the trace (for --trace-summary)")
+    ap.add_argument("--trace-workload", default="", help="what the trace covers (for --trace-summary)")
     args = ap.parse_args()
+    if args.trace_summary:
+        text = trace_summary(args.trace_summary[0], args.trace_command, args.trace_workload)
+        with open(args.trace_summary[1], "w") as f:
+            f.write(text)
+        print(text, end="")
+        return
     workload_tag = "large_fk build 2^%d probe 2^%d passes %d" % (args.log_r, args.log_s, args.passes)
     if args.traffic_from:
         text = traffic_from_pmc(args.traffic_from[0], args.traffic_from[1], workload_tag)
@@ -487,14 +602,63 @@ def main():
     import vjoin
     import workloads
 
+    legs = set(x for x in args.legs.split(",") if x)
+    for off, leg in ((args.no_cpu, "cpu"), (args.no_batched, "batched"), (args.no_serving, "serving"),
+                     (args.no_multicol, "multicol"), (args.no_hash, "hash")):
+        if off:
+            legs.discard(leg)
     torch.cuda.set_device(0)
+    ctx = vjoin.Context(0)
+    ctx.set_profiling(True)
+    out = {}
+    fk = None
+    if "fk" in legs:
+        fk = fk_leg(ctx, args, workload_tag, keep_host="cpu" in legs)
+        out = fk["line"]
+    mc = multicol_data() if ("multicol" in legs or "hash" in legs) else None
+    if "multicol" in legs:
+        print("bench: multi-column skewed join leg", file=sys.stderr, flush=True)
+        out["multicol_skew"] = multicol_leg(ctx, 5, mc)
+    if "hash" in legs:
+        print("bench: hash kernel leg", file=sys.stderr, flush=True)
+        shapes = []
+        if fk:
+            shapes.append(("large_fk probe keys: 2^%d x int64" % args.log_s, [fk["ds"][0]]))
+        shapes.append(("multicol probe keys: 2^25 x 4 uint32", mc[1]))
+        out["hash"] = hash_leg(shapes)
+    if fk:
+        fk.pop("ds", None), fk.pop("dr", None), fk.pop("out", None)
+    mc = None
+    torch.cuda.empty_cache()
+    if "batched" in legs:
+        print("bench: batched leg (%d joins)" % args.batched_joins, file=sys.stderr, flush=True)
+        out["batched"] = batched_leg(ctx, args.batched_joins, args.batched_reps)
+    if "serving" in legs:
+        torch.cuda.empty_cache()
+        print("bench: serving leg (%d requests at %s /s)" % (args.serving_requests, args.serving_rates),
+              file=sys.stderr, flush=True)
+        out["serving"] = serving_leg(args.serving_requests, args.serving_rates)
+    if "cpu" in legs and fk:
+        print("bench: cpu baseline (OpenMP oracle)", file=sys.stderr, flush=True)
+        summ = fk["summary"]
+        out["cpu_baseline"] = cpu_baseline(fk["r_host"], fk["s_host"], summ["passes"], summ["bits1"], summ["bits2"])
+    ctx.close()
+    print(json.dumps(out))
+
+
+def fk_leg(ctx, args, workload_tag, keep_host):
+    """The headline: the large FK join, `args.steps` timed steps after `args.warmup`."""
+    import numpy as np
+    import torch
+
+    import vjoin
+    import workloads
     print("bench: generating large_fk build 2^%d probe 2^%d" % (args.log_r, args.log_s), file=sys.stderr, flush=True)
     rcols, scols = workloads.large_fk(log_r=args.log_r, log_s=args.log_s)
     nr, ns = len(rcols[0]), len(scols[0])
     dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+    r_host, s_host = (rcols[0], scols[0]) if keep_host else (None, None)
     del rcols, scols
-    ctx = vjoin.Context(0)
-    ctx.set_profiling(True)
     build, probe = vjoin.table(dr), vjoin.table(ds)
     if args.no_tune:  # planner bits (profiling runs: no extra joins in the trace)
         opts = vjoin.join_opts("fnv", 64, 64, passes=args.passes)
@@ -521,7 +685,8 @@ def main():
     t_warm = time.perf_counter() - t0
     print("bench: warmup done (%.2f s), first-join scratch %.2f GB" % (t_warm, ctx.memory() / 1e9), file=sys.stderr,
           flush=True)
-    # correctness of the last warm-up output (size-independent property of an FK join)
+    # correctness of the last warm-up output (size-independent property of an FK join:
+    # with unique build keys, every probe row once with an equal key is the whole join)
     sid = out_s.long()
     ok = bool(torch.equal(torch.sort(sid).values, torch.arange(ns, device="cuda"))) and \
         bool(torch.equal(dr[0][out_r.long()], ds[0][sid]))
@@ -554,7 +719,7 @@ def main():
     achieved = sb[dominant] / (float(np.mean(stages[dominant])) * 1e-3)
     B = algorithmic_bytes(nr, ns, m, summ["passes"])
     join_gbs = B / (med["total"] * 1e-3)
-    out = {
+    line = {
         "metric": "probe tuples/s + HBM GB/s (frac of peak); batched vs plain joins/s, p99 latency",
         "value": ns / (ms_step * 1e-3),
         "unit": "probe tuples/s",
@@ -581,30 +746,13 @@ def main():
                           "frac": join_gbs / HBM_PEAK},
         "stage_ms_median": med,
         "stage_GBps": {k: sb[k] / (med[k] * 1e-3) / 1e9 for k in sb if med[k] > 0},
+        "library_summary": {k: summ[k] for k in ("tasks", "partitions", "bytes_total", "gbps_total",
+                                                 "probe_tuples_per_s", "scratch_bytes")},
         "output_alloc_ms": t_alloc * 1e3,
         "parity": {"fk_every_probe_row_once_equal_keys": ok},
     }
-    if not args.no_multicol:
-        print("bench: multi-column skewed join leg", file=sys.stderr, flush=True)
-        out["multicol_skew"] = multicol_leg(ctx, 5)
-    if not args.no_batched:
-        del dr, ds, build, probe, out_r, out_s
-        torch.cuda.empty_cache()
-        print("bench: batched leg (%d joins)" % args.batched_joins, file=sys.stderr, flush=True)
-        out["batched"] = batched_leg(ctx, args.batched_joins, args.batched_reps)
-    if not args.no_serving:
-        if args.no_batched:
-            del dr, ds, build, probe, out_r, out_s
-        torch.cuda.empty_cache()
-        print("bench: serving leg (%d requests at %s /s)" % (args.serving_requests, args.serving_rates),
-              file=sys.stderr, flush=True)
-        out["serving"] = serving_leg(args.serving_requests, args.serving_rates)
-    if not args.no_cpu:
-        print("bench: cpu baseline (OpenMP oracle)", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(args.cpu_log_r, args.cpu_log_s, summ["passes"], summ["bits1"],
-                                           summ["bits2"])
-    ctx.close()
-    print(json.dumps(out))
+    return {"line": line, "summary": summ, "dr": dr, "ds": ds, "out": (out_r, out_s), "r_host": r_host,
+            "s_host": s_host}
 
 
 if __name__ == "__main__":

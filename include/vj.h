@@ -207,6 +207,39 @@ vj_status vj_join_into_noalloc(vj_ctx *ctx, const vj_table *build, const vj_tabl
 
 vj_status vj_pairs_free(vj_pairs *pairs);
 
+/* ------------------------------------------------------------------------ */
+/* The reference's own table layout (db_table, src/base/types.hpp:128-266)   */
+/* ------------------------------------------------------------------------ */
+
+/* A table as the reference stores it: every key column a 64-bit column_t,
+ * row-major (column_values[row * n_cols + c]), and a 64-bit primary key per
+ * row. Device pointers. */
+typedef struct vj_rows {
+    uint64_t n_rows;
+    int32_t n_cols;          /* 1..VJ_MAX_KEY_COLS key columns per row */
+    int32_t reserved;
+    uint64_t *column_values; /* n_rows * n_cols values, row-major */
+    uint64_t *primary_keys;  /* n_rows values */
+} vj_rows;
+
+/*
+ * JoinProvider::join on GPU tables in the reference layout (join_gpu,
+ * src/join/join_provider.cuh:286-421), with the reference's result: `rs` gets
+ * n_cols = 2, column_values[2i] = r->primary_keys[r_row], column_values[2i+1]
+ * = s->primary_keys[s_row] for every matching pair, and primary_keys[i] = i + 1
+ * (copy_probe_results_kernel, src/join/join_provider.cu:593-608;
+ * merge_joined_tables and generate_primary_key_kernel, join_provider.cuh:948-996,
+ * join_provider.cu:307-313). Keys hash as the reference's word stream (each
+ * column's low then high 32-bit word). `rs` is allocated by the callee (as
+ * join_provider.cuh:975-976); release it with vj_rows_free. Pair order is
+ * unspecified. Synchronous; the inputs are not modified. Multi-column rows
+ * are transposed into temporary key columns first (freed before return).
+ */
+vj_status vj_join_rows(vj_ctx *ctx, const vj_rows *r, const vj_rows *s, const vj_join_opts *opts, vj_rows *rs,
+                       void *stream);
+/* Frees a table vj_join_rows allocated (db_table::free, types.hpp:235-261). */
+vj_status vj_rows_free(vj_rows *rows);
+
 /* Per-join statistics of the last join on the context (JoinSummary,
  * join_provider.cuh:54-106; times from hipEvents around each stage). */
 typedef struct vj_join_summary {

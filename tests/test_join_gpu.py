@@ -355,6 +355,66 @@ def test_budget_admission(ctx):
     c2.close()
 
 
+def test_budget_holds_at_the_limit():
+    """Under a budget the arenas take exactly what admission counts: a budget of
+    exactly the join's scratch admits it and the context then holds no more
+    than the budget; one byte less refuses it. vj_join's regrown output is
+    admitted too: a budget that covers the first output but not the regrown
+    one refuses the join instead of exceeding it."""
+    import torch
+    rcols, scols = workloads.plain_small(log_r=14, log_s=18)
+    dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+    opts = vjoin.join_opts(passes=2, bits1=4, bits2=3)
+    want = np_join(workloads.reference_rows(rcols), workloads.reference_rows(scols))
+    o_r = torch.empty(len(scols[0]), dtype=torch.int32, device="cuda")
+    o_s = torch.empty(len(scols[0]), dtype=torch.int32, device="cuda")
+    probe = vjoin.Context(0)
+    probe.set_budget(1 << 40)  # a budget: exact arena sizes
+    rc, n = probe.join_into(vjoin.table(dr), vjoin.table(ds), opts, o_r, o_s)
+    assert rc == 0, probe.last_error()
+    need = probe.memory()
+    probe.close()
+    c = vjoin.Context(0)
+    try:
+        c.set_budget(need)
+        rc, n = c.join_into(vjoin.table(dr), vjoin.table(ds), opts, o_r, o_s)
+        assert rc == 0, c.last_error()
+        assert c.memory() <= need
+        _assert_same(_pairs(o_r[:n], o_s[:n]), want, "at the limit")
+        c2 = vjoin.Context(0)
+        c2.set_budget(need - 1)
+        rc, _ = c2.join_into(vjoin.table(dr), vjoin.table(ds), opts, o_r, o_s)
+        assert vjoin.STATUS[rc] == "VJ_ERR_BUDGET"
+        assert c2.memory() == 0
+        c2.close()
+    finally:
+        c.close()
+    # 1:4 join: the output is 4x the probe side, so vj_join regrows it
+    rng = np.random.default_rng(9)
+    base = rng.permutation(4000).astype(np.uint32)
+    r = np.repeat(base, 4)
+    s = base[rng.integers(0, 4000, 50000)]
+    dr2, ds2 = workloads.to_device([r]), workloads.to_device([s])
+    o1 = vjoin.join_opts(passes=1, bits1=4)
+    probe = vjoin.Context(0)
+    probe.set_budget(1 << 40)
+    probe.join(vjoin.table(dr2), vjoin.table(ds2), o1)
+    scratch = probe.memory()
+    probe.close()
+    c3 = vjoin.Context(0)
+    try:
+        c3.set_budget(scratch + len(s) * 8 * 2)  # room for the first output, not for the 4x one
+        with pytest.raises(vjoin.VJError) as e:
+            c3.join(vjoin.table(dr2), vjoin.table(ds2), o1)
+        assert e.value.name == "VJ_ERR_BUDGET"
+        c3.set_budget(scratch + len(s) * 8 * 4)  # room for the regrown output
+        got = c3.join(vjoin.table(dr2), vjoin.table(ds2), o1)
+        assert c3.summary()["retries"] == 1 and c3.memory() <= scratch
+        _assert_same(_pairs(*got), np_join(r.reshape(-1, 1).astype(np.uint64), s.reshape(-1, 1).astype(np.uint64)))
+    finally:
+        c3.close()
+
+
 def test_large_fk_join_full_size(ctx):
     """2^24 unique int64 build keys, 2^27 FK probe keys, 2 passes: every probe row
     matches exactly one build row with an equal key."""

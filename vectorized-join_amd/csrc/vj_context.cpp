@@ -834,6 +834,33 @@ Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hi
     return Err::ok();
 }
 
+// vj_join's body: a callee-allocated output of the hinted size (or the probe
+// rows), regrown when the join has more pairs. `held_extra` is device memory
+// the caller already holds for this join (counted against the budget).
+Err join_callee_alloc(vj_ctx *c, const vj_table *build, const vj_table *probe, const vj_join_opts *opts, hipStream_t s,
+                      uint64_t held_extra, vj_pairs *out) {
+    const uint64_t cap = opts->out_capacity_hint ? opts->out_capacity_hint : std::max<uint64_t>(probe->n_rows, 1);
+    if (c->budget && cap * 8 + held_extra > c->budget)  // the output alone breaks the budget: nothing is allocated
+        return Err::make(VJ_ERR_BUDGET, "an output of " + std::to_string(cap) + " pairs exceeds the HBM budget");
+    std::vector<JoinSpec> specs{{build, probe, nullptr, nullptr, cap}};
+    if (hipMalloc(&specs[0].out_r, cap * 4) != hipSuccess || hipMalloc(&specs[0].out_s, cap * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        if (specs[0].out_r) (void)hipFree(specs[0].out_r);
+        return Err::make(VJ_ERR_OUT_OF_MEMORY, "output allocation failed");
+    }
+    std::vector<uint64_t> counts;
+    const Err e = run_joins(c, specs, opts, s, counts, true, cap * 8 + held_extra, nullptr);
+    if (e.failed()) {
+        if (specs[0].out_r) (void)hipFree(specs[0].out_r);
+        if (specs[0].out_s) (void)hipFree(specs[0].out_s);
+        return e;
+    }
+    out->count = counts[0];
+    out->build_rowids = specs[0].out_r;
+    out->probe_rowids = specs[0].out_s;
+    return Err::ok();
+}
+
 }  // namespace
 
 namespace vj {
@@ -1057,25 +1084,93 @@ vj_status vj_join(vj_ctx *c, const vj_table *build, const vj_table *probe, const
     if (e.failed()) return fail(c, e);
     e = set_device(c);
     if (e.failed()) return fail(c, e);
-    uint64_t cap = opts->out_capacity_hint ? opts->out_capacity_hint : std::max<uint64_t>(probe->n_rows, 1);
-    if (c->budget && cap * 8 > c->budget)  // the output alone breaks the budget: nothing is allocated
-        return fail(c, Err::make(VJ_ERR_BUDGET, "an output of " + std::to_string(cap) + " pairs exceeds the HBM budget"));
-    std::vector<JoinSpec> specs{{build, probe, nullptr, nullptr, cap}};
-    if (hipMalloc(&specs[0].out_r, cap * 4) != hipSuccess || hipMalloc(&specs[0].out_s, cap * 4) != hipSuccess) {
-        (void)hipGetLastError();
-        if (specs[0].out_r) (void)hipFree(specs[0].out_r);
-        return fail(c, Err::make(VJ_ERR_OUT_OF_MEMORY, "output allocation failed"));
+    e = join_callee_alloc(c, build, probe, opts, (hipStream_t)stream, 0, out);
+    return e.failed() ? fail(c, e) : VJ_OK;
+}
+
+vj_status vj_join_rows(vj_ctx *c, const vj_rows *r, const vj_rows *s, const vj_join_opts *opts, vj_rows *rs,
+                       void *stream) {
+    if (!c || !r || !s || !rs) return VJ_ERR_INVALID_ARGUMENT;
+    c->last_error.clear();
+    memset(rs, 0, sizeof(*rs));
+    vj_join_opts d;
+    vj_join_opts_default(&d);
+    if (!opts) opts = &d;
+    // join_gpu asserts equal column counts (join_provider.cuh:291)
+    if (r->n_cols != s->n_cols || r->n_cols < 1 || r->n_cols > VJ_MAX_KEY_COLS)
+        return fail(c, Err::make(VJ_ERR_INVALID_ARGUMENT, "both tables need the same number (1..4) of key columns"));
+    for (const vj_rows *t : {r, s}) {
+        if (t->n_rows > VJ_MAX_ROWS)
+            return fail(c, Err::make(VJ_ERR_UNSUPPORTED, "tables must have at most 2^32 - 2^16 rows"));
+        if (t->n_rows && (!t->column_values || !t->primary_keys))
+            return fail(c, Err::make(VJ_ERR_INVALID_ARGUMENT, "column_values / primary_keys is NULL"));
     }
-    std::vector<uint64_t> counts;
-    e = run_joins(c, specs, opts, (hipStream_t)stream, counts, true, cap * 8, nullptr);
+    Err e = set_device(c);
+    if (e.failed()) return fail(c, e);
+    hipStream_t st = (hipStream_t)stream;
+    const int nc = r->n_cols;
+    // Key columns for the pipeline: one column of a row-major table is already
+    // contiguous; wider rows are transposed into temporary columns.
+    uint64_t *tmp = nullptr;
+    const uint64_t tmp_words = nc > 1 ? (r->n_rows + s->n_rows) * (uint64_t)nc : 0;
+    if (tmp_words) {
+        if (c->budget && held_bytes(c) + tmp_words * 8 > c->budget)
+            return fail(c, Err::make(VJ_ERR_BUDGET, "the transposed key columns exceed the HBM budget"));
+        if (hipMalloc(&tmp, tmp_words * 8) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(c, Err::make(VJ_ERR_OUT_OF_MEMORY, "hipMalloc of the transposed key columns failed"));
+        }
+        e = launch_rows_to_cols(r->column_values, r->n_rows, nc, tmp, st, c->cus);
+        if (!e.failed()) e = launch_rows_to_cols(s->column_values, s->n_rows, nc, tmp + r->n_rows * nc, st, c->cus);
+    }
+    vj_table rt{}, stt{};
+    for (int k = 0; k < 2; k++) {
+        const vj_rows *src = k ? s : r;
+        vj_table &t = k ? stt : rt;
+        t.n_rows = src->n_rows;
+        t.n_cols = nc;
+        t.key_width = 8;
+        uint64_t *base = nc > 1 ? tmp + (k ? r->n_rows * nc : 0) : src->column_values;
+        for (int cc = 0; cc < nc; cc++) t.cols[cc] = nc > 1 ? base + (uint64_t)cc * src->n_rows : base;
+    }
+    vj_pairs pairs{};
+    if (!e.failed()) e = join_callee_alloc(c, &rt, &stt, opts, st, tmp_words * 8, &pairs);
+    const uint64_t m = pairs.count;
+    if (!e.failed() && m) {  // rs: (r pk, s pk) per pair, row-major, and primary keys 1..M
+        if (c->budget && held_bytes(c) + tmp_words * 8 + m * 8 + m * 24 > c->budget) {
+            e = Err::make(VJ_ERR_BUDGET, "the joined table exceeds the HBM budget");
+        } else if (hipMalloc(&rs->column_values, m * 16) != hipSuccess ||
+                   hipMalloc(&rs->primary_keys, m * 8) != hipSuccess) {
+            (void)hipGetLastError();
+            e = Err::make(VJ_ERR_OUT_OF_MEMORY, "hipMalloc of the joined table failed");
+        } else {
+            e = launch_materialize(pairs.build_rowids, pairs.probe_rowids, m, r->primary_keys, s->primary_keys,
+                                   rs->column_values, rs->primary_keys, st, c->cus);
+        }
+    }
+    if (!e.failed()) {
+        const hipError_t he = hipStreamSynchronize(st);
+        if (he != hipSuccess) e = Err::make(VJ_ERR_HIP, std::string("vj_join_rows: ") + hipGetErrorString(he));
+    } else {
+        (void)hipStreamSynchronize(st);  // nothing queued may still use what is freed below
+    }
+    (void)vj_pairs_free(&pairs);
+    if (tmp) (void)hipFree(tmp);
     if (e.failed()) {
-        if (specs[0].out_r) (void)hipFree(specs[0].out_r);
-        if (specs[0].out_s) (void)hipFree(specs[0].out_s);
+        (void)vj_rows_free(rs);
         return fail(c, e);
     }
-    out->count = counts[0];
-    out->build_rowids = specs[0].out_r;
-    out->probe_rowids = specs[0].out_s;
+    rs->n_rows = m;
+    rs->n_cols = 2;
+    return VJ_OK;
+}
+
+vj_status vj_rows_free(vj_rows *t) {
+    if (!t) return VJ_ERR_INVALID_ARGUMENT;
+    if (t->column_values) (void)hipFree(t->column_values);
+    if (t->primary_keys) (void)hipFree(t->primary_keys);
+    t->column_values = t->primary_keys = nullptr;
+    t->n_rows = 0;
     return VJ_OK;
 }
 

@@ -15,19 +15,25 @@ constexpr int kHashThreads = 256;
 constexpr int kHashUnroll = 4;
 
 template <int HP, int KW, int NC, typename OutT>
-__global__ __launch_bounds__(kHashThreads) void k_hash(TView t, HashCfg h, OutT *__restrict__ out) {
+__global__ __launch_bounds__(kHashThreads) void k_hash(TView t, HashCfg h, OutT *out_) {
+    // key columns and output as global-address-space pointers (global_load /
+    // global_store, counted on vmcnt only), copied out of the view once
+    GKeys<KW, NC> in;
+#pragma unroll
+    for (int c = 0; c < NC; c++) in.c[c] = to_global(static_cast<const typename KeyT<KW>::T *>(t.cols[c]));
+    const gptr<OutT> out = to_global(out_);
     const uint64_t n = t.n;
     const uint64_t stride = (uint64_t)gridDim.x * kHashThreads * kHashUnroll;
     for (uint64_t base = (uint64_t)blockIdx.x * kHashThreads * kHashUnroll + threadIdx.x; base < n; base += stride) {
         Key<KW, NC> k[kHashUnroll];
 #pragma unroll
-        for (int u = 0; u < kHashUnroll; u++) {
-            uint64_t i = base + (uint64_t)u * kHashThreads;
-            if (i < n) k[u] = load_key<KW, NC>(t, i);
+        for (int u = 0; u < kHashUnroll; u++) {  // all loads in flight at once (past the end: row 0 again)
+            const uint64_t i = base + (uint64_t)u * kHashThreads;
+            k[u] = load_key<KW, NC>(in, i < n ? i : 0);
         }
 #pragma unroll
         for (int u = 0; u < kHashUnroll; u++) {
-            uint64_t i = base + (uint64_t)u * kHashThreads;
+            const uint64_t i = base + (uint64_t)u * kHashThreads;
             if (i < n) out[i] = (OutT)hash_key<HP, KW, NC>(k[u], h);
         }
     }

@@ -45,6 +45,11 @@ TView to_view(const vj_table *t);
 // vj_hash.hip
 Err launch_hash(const vj_table *t, const HashCfg &cfg, void *out, hipStream_t s, int num_cus);
 
+// vj_rows.hip: the reference's row-major table layout
+Err launch_rows_to_cols(const uint64_t *rows, uint64_t n, int ncols, uint64_t *cols, hipStream_t s, int cus);
+Err launch_materialize(const uint32_t *ri, const uint32_t *si, uint64_t m, const uint64_t *r_pk, const uint64_t *s_pk,
+                       uint64_t *rs_cols, uint64_t *rs_pk, hipStream_t s, int cus);
+
 int device_cu_count(int device);
 
 // vj_noalloc.hip: the fixed device scratch of vj_join_into_noalloc.

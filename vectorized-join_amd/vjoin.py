@@ -48,6 +48,12 @@ class JoinOpts(ctypes.Structure):
                 ("bits2", ctypes.c_int32), ("one_based", ctypes.c_int32), ("out_capacity_hint", ctypes.c_uint64)]
 
 
+class Rows(ctypes.Structure):
+    """vj_rows: a table in the reference layout (row-major uint64 key columns + uint64 primary keys)."""
+    _fields_ = [("n_rows", ctypes.c_uint64), ("n_cols", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("column_values", ctypes.c_void_p), ("primary_keys", ctypes.c_void_p)]
+
+
 class Pairs(ctypes.Structure):
     _fields_ = [("count", ctypes.c_uint64), ("build_rowids", ctypes.c_void_p), ("probe_rowids", ctypes.c_void_p)]
 
@@ -116,6 +122,8 @@ def lib():
             "vj_join_into": ([vp, P(Table), P(Table), P(JoinOpts), vp, vp, u64, P(u64), vp], i32),
             "vj_join_into_noalloc": ([vp, P(Table), P(Table), P(JoinOpts), vp, vp, u64, P(u64), vp], i32),
             "vj_pairs_free": ([P(Pairs)], i32),
+            "vj_join_rows": ([vp, P(Rows), P(Rows), P(JoinOpts), P(Rows), vp], i32),
+            "vj_rows_free": ([P(Rows)], i32),
             "vj_join_summary_get": ([vp, P(JoinSummary)], i32),
             "vj_partition": ([vp, P(Table), P(HashOpts), i32, i32, P(Table), vp, vp, vp], i32),
             "vj_join_batch": ([vp, i32, P(Table), P(Table), P(JoinOpts), P(vp), P(vp), P(u64), P(u64), i32, vp],
@@ -274,6 +282,34 @@ class Context:
             lib().vj_pairs_free(ctypes.byref(p))
         return r, s
 
+    def join_rows(self, r_values, r_pks, s_values, s_pks, opts):
+        """vj_join_rows on reference-layout tables (device int64 tensors: values [n, cols] row-major, pks [n]).
+        Returns (rs_values [m, 2], rs_pks [m]) as torch int64 tensors (copies; frees the callee's table)."""
+        import torch
+
+        def rows(v, pk):
+            t = Rows()
+            t.n_rows = int(pk.numel())
+            t.n_cols = int(v.shape[1])
+            t.column_values = v.data_ptr() if t.n_rows else None
+            t.primary_keys = pk.data_ptr() if t.n_rows else None
+            return t
+        r, s, rs = rows(r_values, r_pks), rows(s_values, s_pks), Rows()
+        check(lib().vj_join_rows(self.ptr, ctypes.byref(r), ctypes.byref(s), ctypes.byref(opts), ctypes.byref(rs),
+                                 stream_ptr()), "vj_join_rows", self.ptr)
+        try:
+            m = int(rs.n_rows)
+            assert m == 0 or rs.n_cols == 2
+            vals = torch.empty((m, 2), dtype=torch.int64, device="cuda")
+            pks = torch.empty(m, dtype=torch.int64, device="cuda")
+            if m:
+                _copy_dev(vals, rs.column_values, m * 16)
+                _copy_dev(pks, rs.primary_keys, m * 8)
+            torch.cuda.synchronize()
+        finally:
+            lib().vj_rows_free(ctypes.byref(rs))
+        return vals, pks
+
     def summary(self):
         s = JoinSummary()
         check(lib().vj_join_summary_get(self.ptr, ctypes.byref(s)), "vj_join_summary_get", self.ptr)
@@ -413,7 +449,8 @@ class _DevArray:
 
 
 def _copy_dev(dst_tensor, src_ptr, nbytes):
+    """Byte copy of nbytes (a multiple of 4) from a raw device pointer into a contiguous tensor."""
     import torch
     n = nbytes // 4
     view = torch.as_tensor(_DevArray(src_ptr, n), device=dst_tensor.device)
-    dst_tensor.copy_(view)
+    dst_tensor.view(-1).view(torch.int32)[:n].copy_(view)

@@ -153,10 +153,10 @@ def traffic_from_pmc(fetch_csv, write_csv, workload):
     import numpy as np
     lines = ["# HBM traffic per launch, from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch),",
              "# each scaled by counter(1 GiB device copy) / 1 GiB. Workload: " + workload,
-             "# made by: rocprofv3 --pmc FETCH_SIZE -d F -o run -- python3 bench.py --steps 1 --warmup 1 --no-cpu "
-             "--no-batched --no-serving --no-tune --no-multicol --calib-copy",
-             "#          rocprofv3 --pmc WRITE_SIZE -d W -o run -- python3 bench.py --steps 1 --warmup 1 --no-cpu "
-             "--no-batched --no-serving --no-tune --no-multicol --calib-copy",
+             "# made by: rocprofv3 --pmc FETCH_SIZE -d F -o run -- python3 bench.py --legs fk --steps 1 --warmup 1 "
+             "--no-tune --calib-copy",
+             "#          rocprofv3 --pmc WRITE_SIZE -d W -o run -- python3 bench.py --legs fk --steps 1 --warmup 1 "
+             "--no-tune --calib-copy",
              "#          python3 bench.py --traffic-from F/run_results.db W/run_results.db"]
     factors, per = {}, {}
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
@@ -445,12 +445,110 @@ def serving_leg(n_requests, rates, budget_gb=64, n_pool=64, n_tables=8, n_reprob
         torch.cuda.empty_cache()
     st = srv.stats()
     results["server_stats"] = {k: st[k] for k in ("peak_bytes", "resident_tables", "resident_bytes", "max_batch",
-                                                  "budget_waits")}
+                                                  "budget_waits", "budget_deferred")}
     srv.close()
     for b in built:
         b.free()
     ctx.close()
     return results
+
+
+def serving_budget_leg(budget_gb=64, n_requests=48, seed=11):
+    """A burst of large requests that together need far more than the HBM
+    budget: 60 % fresh FK joins (build 2^22..2^24 int64 keys, probe 8x, every
+    probe row matching once), 40 % re-probes of two resident 2^24-row tables
+    with 2^26-row FK probes, all submitted at once. The server admits what fits
+    the budget (scratch of the launch + the other lane's arenas + resident
+    tables) and holds the rest back until memory frees up; the plain loop runs
+    the same requests one join per launch. Every request's pair count and
+    checksum must agree between the two."""
+    import numpy as np
+    import torch
+
+    import vjoin
+    import workloads
+    rng = np.random.default_rng(seed)
+    pool = []
+    for k, (lr, ls) in enumerate(((22, 25), (23, 26), (24, 27), (24, 26))):
+        rcols, scols = workloads.large_fk(log_r=lr, log_s=ls, seed=seed * 100 + k)
+        dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+        pool.append((vjoin.table(dr), vjoin.table(ds), 1 << ls, (dr, ds)))
+    res_src = []
+    for k in range(2):
+        rcols, _ = workloads.large_fk(log_r=24, log_s=10, seed=seed * 100 + 50 + k)
+        keys = rcols[0]
+        s = keys[rng.integers(0, len(keys), size=1 << 26)]
+        dr, ds = workloads.to_device(rcols), workloads.to_device([s])
+        res_src.append((vjoin.table(dr), vjoin.table(ds), 1 << 26, (dr, ds)))
+    opts = vjoin.join_opts("fnv", 64, 64)
+    budget = int(budget_gb * (1 << 30))
+    srv = vjoin.Server(0, budget=budget, opts=opts, max_batch=64)
+    ctx = vjoin.Context(0)
+    tid, built = [], []
+    for b, _, _, keep in res_src:
+        o = (torch.empty(1024, dtype=torch.int32, device="cuda"), torch.empty(1024, dtype=torch.int32, device="cuda"))
+        probe = vjoin.table([keep[0][0][:1024]])
+        r = srv.wait(srv.submit(probe, o[0], o[1], build=b, keep_build=True))
+        assert r["status"] == 0 and r["table"] > 0, r
+        tid.append(r["table"])
+        built.append(ctx.build(b, opts))
+    kinds = [("fresh", int(rng.integers(0, len(pool)))) if rng.random() < 0.6 else ("re", int(rng.integers(0, 2)))
+             for _ in range(n_requests)]
+    outs = []
+    for kind, j in kinds:
+        n = pool[j][2] if kind == "fresh" else res_src[j][2]
+        outs.append((torch.empty(n, dtype=torch.int32, device="cuda"), torch.empty(n, dtype=torch.int32, device="cuda")))
+    torch.cuda.synchronize()
+    st0 = srv.stats()
+    t0 = time.monotonic_ns()
+    tickets = []
+    for (kind, j), o in zip(kinds, outs):
+        if kind == "fresh":
+            tickets.append(srv.submit(pool[j][1], o[0], o[1], build=pool[j][0]))
+        else:
+            tickets.append(srv.submit(res_src[j][1], o[0], o[1], table=tid[j]))
+    resp = [srv.wait(t) for t in tickets]
+    st1 = srv.stats()
+    assert all(r["status"] == 0 for r in resp), [r for r in resp if r["status"]][:3]
+    lat = np.array([(r["t_done_ns"] - t0) * 1e-6 for r in resp])
+    span = (max(r["t_done_ns"] for r in resp) - t0) * 1e-9
+    served_cnt = [r["count"] for r in resp]
+    served_sum = _pair_checksums([o[0] for o in outs], [o[1] for o in outs], served_cnt)
+    for o in outs:
+        o[0].fill_(-1)
+        o[1].fill_(-1)
+    torch.cuda.synchronize()
+    plain_cnt, plain_lat = [], []
+    t0p = time.perf_counter()
+    for (kind, j), o in zip(kinds, outs):
+        if kind == "fresh":
+            rc, cnt = ctx.join_into(pool[j][0], pool[j][1], opts, o[0], o[1])
+        else:
+            rc, cnt = ctx.probe_into(built[j], res_src[j][1], opts, o[0], o[1])
+        assert rc == 0
+        plain_cnt.append(cnt)
+        plain_lat.append((time.perf_counter() - t0p) * 1e3)
+    plain_span = time.perf_counter() - t0p
+    plain_sum = _pair_checksums([o[0] for o in outs], [o[1] for o in outs], plain_cnt)
+    st = srv.stats()
+    out = {"workload": "burst of %d requests: 60%% fresh FK joins (build 2^22..2^24 int64 keys, probe 8x), 40%% "
+                       "re-probes of 2 resident 2^24-row tables with 2^26-row probes" % n_requests,
+           "budget_GB": budget_gb,
+           "served": {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                      "joins_per_s": n_requests / span, "batches": st1["batches"] - st0["batches"],
+                      "budget_waits": st1["budget_waits"] - st0["budget_waits"],
+                      "budget_deferred": st1["budget_deferred"] - st0["budget_deferred"]},
+           "plain": {"p50_ms": float(np.percentile(plain_lat, 50)), "p99_ms": float(np.percentile(plain_lat, 99)),
+                     "joins_per_s": n_requests / plain_span},
+           "peak_bytes": st["peak_bytes"], "peak_within_budget": st["peak_bytes"] <= budget,
+           "same_counts_and_checksums": served_cnt == plain_cnt and served_sum == plain_sum}
+    srv.close()
+    for b in built:
+        b.free()
+    ctx.close()
+    del outs, pool, res_src
+    torch.cuda.empty_cache()
+    return out
 
 
 def multicol_data():
@@ -630,6 +728,13 @@ def main():
         fk.pop("ds", None), fk.pop("dr", None), fk.pop("out", None)
     mc = None
     torch.cuda.empty_cache()
+    if args.calib_copy:  # a plain 1 GiB device copy, three times: the scale of FETCH_SIZE / WRITE_SIZE
+        x = torch.empty(CALIB_BYTES // 8, dtype=torch.int64, device="cuda").fill_(1)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
     if "batched" in legs:
         print("bench: batched leg (%d joins)" % args.batched_joins, file=sys.stderr, flush=True)
         out["batched"] = batched_leg(ctx, args.batched_joins, args.batched_reps)
@@ -638,6 +743,9 @@ def main():
         print("bench: serving leg (%d requests at %s /s)" % (args.serving_requests, args.serving_rates),
               file=sys.stderr, flush=True)
         out["serving"] = serving_leg(args.serving_requests, args.serving_rates)
+        torch.cuda.empty_cache()
+        print("bench: serving leg under the 64 GB budget", file=sys.stderr, flush=True)
+        out["serving_budget"] = serving_budget_leg()
     if "cpu" in legs and fk:
         print("bench: cpu baseline (OpenMP oracle)", file=sys.stderr, flush=True)
         summ = fk["summary"]
@@ -705,13 +813,6 @@ def fk_leg(ctx, args, workload_tag, keep_host):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     summ = ctx.summary()
-    if args.calib_copy:
-        x = torch.empty(CALIB_BYTES // 8, dtype=torch.int64, device="cuda").fill_(1)
-        y = torch.empty_like(x)
-        for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        del x, y
     ms_step = elapsed / args.steps * 1e3
     med = {k: float(np.median(v)) for k, v in stages.items()}
     sb = stage_bytes(nr, ns, m)

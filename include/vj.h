@@ -369,6 +369,7 @@ typedef struct vj_server_stats {
     uint64_t resident_tables, resident_bytes;
     uint64_t peak_bytes;         /* most device memory held (arenas + resident tables) */
     uint64_t budget_waits;       /* times the head request waited for memory */
+    uint64_t budget_deferred;    /* launches that left queued requests behind because the budget was reached */
 } vj_server_stats;
 
 /* Starts the engine on `device` with a worker thread and two launch lanes

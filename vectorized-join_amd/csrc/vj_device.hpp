@@ -153,9 +153,11 @@ template <typename HT> __device__ __forceinline__ HT shl_clamp(HT x, int s) {
 template <typename HT> struct Fnv;
 template <> struct Fnv<uint32_t> {
     static constexpr uint32_t prime = 16777619u, offset = 2166136261u;
+    static constexpr uint32_t prime2 = prime * prime;  // mod 2^32
 };
 template <> struct Fnv<uint64_t> {
     static constexpr uint64_t prime = 1099511628211ull, offset = 14695981039346656037ull;
+    static constexpr uint64_t prime2 = prime * prime;  // mod 2^64
 };
 
 // get_max_shift(), hash.cu:60-62, with sizeof(chunk_t) from chunk_bits.
@@ -317,9 +319,52 @@ __device__ __noinline__ HT generic_words(const uint32_t (&w)[NW], int n, int kin
     return s.hash;
 }
 
+// The same two fast paths for 4-byte key columns widened by zero extension
+// (reference words, unsigned keys): every column contributes the words (k, 0).
+// FNV-1a: ((h ^ k) * P ^ 0) * P = (h ^ k) * P^2 (mod 2^bits), one multiply per
+// column. custom_xor_hw: the zero word adds 0 to the shift, xors in
+// rotl(0, s) = 0 and reverses the bits again, undoing the reversal of the
+// column's own word, so each column is h ^= rotl(k, s) with s += k. Both equal
+// the word-by-word results exactly.
+template <typename HT, int NC> __device__ __forceinline__ HT fnv_zext_cols(const uint32_t (&k)[NC]) {
+    HT h = Fnv<HT>::offset;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        h ^= k[c];
+        h *= Fnv<HT>::prime2;
+    }
+    return h;
+}
+template <typename HT, int NC> __device__ __forceinline__ HT xor_hw_zext_cols(const uint32_t (&k)[NC]) {
+    HT h = 0;
+    uint32_t shift = 0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        shift += k[c];
+        h ^= (HT)permutate(k[c], (int)shift);
+    }
+    return h;
+}
+
 // Hash of a key as a uint64 (32-bit hashes zero-extended).
 template <int HP, int KW, int NC>
 __device__ __forceinline__ uint64_t hash_key(const Key<KW, NC> &k, const HashCfg &h) {
+    // 4-byte keys on the fast paths: the word count is fixed by the (wave-uniform)
+    // word mode, so the words are not assembled at run-time positions
+    if (KW == 4 && HP != HP_GENERIC && !h.key_signed) {
+        uint32_t c[NC];
+#pragma unroll
+        for (int i = 0; i < NC; i++) c[i] = (uint32_t)k.c[i];
+        if (!h.native) {  // reference words (k, 0) per column
+            if (HP == HP_FNV)
+                return h.bits == 32 ? (uint64_t)fnv_zext_cols<uint32_t, NC>(c) : fnv_zext_cols<uint64_t, NC>(c);
+            return h.bits == 32 ? (uint64_t)xor_hw_zext_cols<uint32_t, NC>(c) : xor_hw_zext_cols<uint64_t, NC>(c);
+        }
+        // native words: one per column
+        if (HP == HP_FNV)
+            return h.bits == 32 ? (uint64_t)fnv_words<uint32_t, NC>(c, NC) : fnv_words<uint64_t, NC>(c, NC);
+        return h.bits == 32 ? (uint64_t)xor_hw_words<uint32_t, NC>(c, NC) : xor_hw_words<uint64_t, NC>(c, NC);
+    }
     uint32_t w[2 * NC];
     int n = key_words<KW, NC>(k, h, w);
     if (HP == HP_FNV) {

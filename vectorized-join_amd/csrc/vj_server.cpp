@@ -7,8 +7,10 @@
 // and kept tables) and launches them as one pipeline (every stage one launch
 // over the tiles of all admitted joins, one persistent build+probe launch).
 // A batch is launched while the other lane's batch is still on the device, so
-// requests that arrive during a launch start before it ends. Completion is
-// detected by polling the lanes' events.
+// requests that arrive during a launch start before it ends. A request that
+// arrives at an idle server is launched by the submitting thread itself.
+// Completion is detected by the worker, which polls the lanes' events back to
+// back while a batch is on the device and sleeps when nothing is.
 //
 // Reference counterpart: JoinProvider::probe (src/join/join_provider.cuh:792-911),
 // which runs bucket pairs on up to 50 detached host threads over round-robin
@@ -61,6 +63,11 @@ struct Lane {
     hipStream_t s = nullptr;
     hipEvent_t ev = nullptr;
     bool busy = false;
+    // set while a thread plans and launches this lane's batch with the lock
+    // released; the lane's event is recorded only at the end of that, so until
+    // then the event still reports the previous batch (or, never recorded,
+    // reports complete) and must not be polled
+    bool launching = false;
     std::vector<Req *> reqs;
     size_t count_off = 0;
     uint64_t kept_bytes = 0;
@@ -90,6 +97,9 @@ struct vj_server {
     uint64_t next_ticket = 1, next_table = 1;
     uint64_t resident_bytes = 0;
     Lane lanes[2];
+    // one launch at a time (from the worker or a submitting thread): admission
+    // reads the other lane's arena sizes, which only a launch changes
+    bool launch_in_progress = false;
     std::thread worker;
     bool stop = false;
     vj_server_stats st{};
@@ -173,13 +183,18 @@ LaunchResult launch_batch(vj_server *sv, Lane &L, Lane &O, std::unique_lock<std:
     }
     std::vector<Req *> batch;
     uint64_t need = 0;
+    bool budget_cut = false;
     while (!sv->pending.empty() && (int)batch.size() < sv->max_batch) {
         Req *r = sv->pending.front();
-        if (avail && need + r->need > avail) break;
+        if (avail && need + r->need > avail) {
+            budget_cut = true;
+            break;
+        }
         need += r->need;
         batch.push_back(r);
         sv->pending.pop_front();
     }
+    if (budget_cut && !batch.empty()) sv->st.budget_deferred++;
     if (batch.empty()) {
         Req *r = sv->pending.front();
         if (r->need + sv->resident_bytes > sv->budget) {  // cannot fit even with both lanes empty
@@ -201,6 +216,8 @@ LaunchResult launch_batch(vj_server *sv, Lane &L, Lane &O, std::unique_lock<std:
         if (r->keep) r->kept = new vj_built();
     }
     L.busy = true;
+    L.launching = true;
+    sv->launch_in_progress = true;
     L.reqs = batch;
     sv->st.batches++;
     if (O.busy) sv->st.overlapped_batches++;
@@ -220,6 +237,8 @@ LaunchResult launch_batch(vj_server *sv, Lane &L, Lane &O, std::unique_lock<std:
     // requests are reported failed (and their buffers can be freed).
     if (e.failed()) (void)hipStreamSynchronize(L.s);
     lk.lock();
+    L.launching = false;
+    sv->launch_in_progress = false;
     for (Req *r : batch)
         if (r->kept) L.kept_bytes += r->kept->bytes;
     const uint64_t held = held_now(sv);
@@ -228,13 +247,23 @@ LaunchResult launch_batch(vj_server *sv, Lane &L, Lane &O, std::unique_lock<std:
     return LAUNCHED;
 }
 
+// Launches queued requests on a free lane, if any (with sv->m held; the lock
+// is released during the HIP calls). Returns true if something happened.
+bool try_launch(vj_server *sv, std::unique_lock<std::mutex> &lk) {
+    if (sv->pending.empty() || sv->launch_in_progress) return false;
+    Lane *F = !sv->lanes[0].busy ? &sv->lanes[0] : (!sv->lanes[1].busy ? &sv->lanes[1] : nullptr);
+    if (!F) return false;
+    Lane &O = F == &sv->lanes[0] ? sv->lanes[1] : sv->lanes[0];
+    return launch_batch(sv, *F, O, lk) != WAITING;
+}
+
 void worker_main(vj_server *sv) {
     (void)hipSetDevice(sv->device);
     std::unique_lock<std::mutex> lk(sv->m);
     while (true) {
         bool progressed = false;
         for (Lane &L : sv->lanes) {
-            if (!L.busy) continue;
+            if (!L.busy || L.launching) continue;
             const hipError_t q = hipEventQuery(L.ev);
             if (q == hipSuccess) {
                 complete_lane(sv, L, VJ_OK);
@@ -245,20 +274,19 @@ void worker_main(vj_server *sv) {
                 progressed = true;
             }
         }
-        if (!sv->pending.empty()) {
-            Lane *F = !sv->lanes[0].busy ? &sv->lanes[0] : (!sv->lanes[1].busy ? &sv->lanes[1] : nullptr);
-            if (F) {
-                Lane &O = F == &sv->lanes[0] ? sv->lanes[1] : sv->lanes[0];
-                if (launch_batch(sv, *F, O, lk) != WAITING) progressed = true;
-            }
-        }
+        if (try_launch(sv, lk)) progressed = true;
         const bool busy = sv->lanes[0].busy || sv->lanes[1].busy;
         if (sv->stop && sv->pending.empty() && !busy) break;
         if (progressed) continue;
-        if (busy)
-            sv->cv_work.wait_for(lk, std::chrono::microseconds(20));
-        else
+        if (busy) {
+            // a batch is on the device: poll its event again at once (a timed
+            // wait here would add its period to every request's latency)
+            lk.unlock();
+            std::this_thread::yield();
+            lk.lock();
+        } else {
             sv->cv_work.wait(lk);
+        }
     }
 }
 
@@ -340,6 +368,16 @@ vj_status vj_server_submit(vj_server *sv, const vj_request *rq, uint64_t *ticket
     sv->pending.push_back(r);
     sv->st.submitted++;
     *ticket = r->ticket;
+    // Light load: the queue held only this request and a lane is free, so the
+    // submitting thread launches it at once instead of handing it to the
+    // worker (whose wake-up would add to the request's latency). The worker
+    // is woken either way: it detects the completion.
+    if (sv->pending.size() == 1 && !sv->stop) {
+        int prev = -1;
+        const bool have_prev = hipGetDevice(&prev) == hipSuccess;
+        (void)try_launch(sv, lk);                                     // switches to the server's device
+        if (have_prev && prev != sv->device) (void)hipSetDevice(prev);  // the caller's thread keeps its device
+    }
     lk.unlock();
     sv->cv_work.notify_one();
     return VJ_OK;

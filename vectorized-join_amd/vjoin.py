@@ -73,7 +73,7 @@ class Response(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "failed", "batches", "overlapped_batches",
                                                 "max_batch", "resident_tables", "resident_bytes", "peak_bytes",
-                                                "budget_waits")]
+                                                "budget_waits", "budget_deferred")]
 
 
 class JoinSummary(ctypes.Structure):

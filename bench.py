@@ -13,11 +13,13 @@ average duration, from HIP events the library records on the join's stream
 around each stage. Further legs, in the same line:
   multicol_skew  the multi-column Zipf join (2^22 x 2^25, 4 x uint32 keys)
   hash           the standalone hash kernel over HASH_BITS x chunk widths
+  bigbuild       a 2^26 x 2^27 FK join, wide vs. standard build+probe block
   batched        256 uneven joins, one batched pipeline vs. sequential launches
   serving        a Poisson request stream, served vs. one join per launch
   cpu_baseline   the same algorithm (the OpenMP oracle) on the host cores
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs fk,multicol,hash,batched,serving,cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+                       [--legs fk,multicol,hash,bigbuild,batched,serving,cpu]
 """
 import argparse
 import json
@@ -605,6 +607,53 @@ def multicol_leg(ctx, steps, data):
     return res
 
 
+def bigbuild_leg(ctx, steps=5, log_r=26, log_s=27):
+    """A build side past the standard plan: 2^26 unique int64 keys (FK probe of
+    2^27 rows), FNV-1a-64. With 2048-row tables it would need 16 radix bits,
+    past two passes' 14; the planner's wide build+probe block (1024 threads,
+    8192-row tables) keeps one build chunk per partition at 14 bits. Timed
+    against the same join forced onto the standard block (several build chunks
+    per partition, so every probe chunk is read once per build chunk). Checked
+    as the FK join: every probe row once, with an equal key."""
+    import numpy as np
+    import torch
+
+    import vjoin
+    import workloads
+    rcols, scols = workloads.large_fk(log_r=log_r, log_s=log_s, seed=26)
+    dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+    del rcols, scols
+    nr, ns = int(dr[0].numel()), int(ds[0].numel())
+    out_r = torch.empty(ns, dtype=torch.int32, device="cuda")
+    out_s = torch.empty(ns, dtype=torch.int32, device="cuda")
+    res = {"workload": "build 2^%d unique int64 keys, FK probe 2^%d rows, FNV-1a-64, 2 radix passes" % (log_r, log_s),
+           "build_rows": nr, "probe_rows": ns}
+    for name, table in (("wide_table", 0), ("standard_table", 1)):
+        opts = vjoin.join_opts("fnv", 64, 64, table=table)
+        b, p = vjoin.table(dr), vjoin.table(ds)
+        ms, stages = [], {k: [] for k in ("ms_histogram", "ms_pass1", "ms_pass2", "ms_build_probe")}
+        for i in range(steps + 1):
+            rc, cnt = ctx.join_into(b, p, opts, out_r, out_s)
+            if rc != 0:
+                raise RuntimeError("join failed: %s %s" % (vjoin.STATUS.get(rc), ctx.last_error()))
+            if i:
+                sm = ctx.summary()
+                ms.append(sm["ms_total"])
+                for k in stages:
+                    stages[k].append(sm[k])
+        sid = out_s.long()
+        ok = cnt == ns and bool(torch.equal(torch.sort(sid).values, torch.arange(ns, device="cuda"))) and \
+            bool(torch.equal(dr[0][out_r.long()], ds[0][sid]))
+        del sid
+        t = float(np.median(ms))
+        B = algorithmic_bytes(nr, ns, ns, sm["passes"])
+        res[name] = {"ms": t, "stage_ms": {k[3:]: float(np.median(v)) for k, v in stages.items()},
+                     "passes": sm["passes"], "bits": [sm["bits1"], sm["bits2"]], "probe_threads": sm["probe_threads"],
+                     "table_rows": sm["table_rows"], "tasks": sm["tasks"], "probe_tuples_per_s": ns / (t * 1e-3),
+                     "algorithmic_bytes": B, "frac": B / (t * 1e-3) / HBM_PEAK, "parity": bool(ok)}
+    return res
+
+
 def hash_leg(shapes, reps=10):
     """vj_hash (the standalone hash kernel, k_hash) over the probe-side keys of the
     large FK join and of the multi-column join, for FNV-1a and custom_xor_hw,
@@ -654,8 +703,9 @@ def main():
     ap.add_argument("--log-r", type=int, default=24)
     ap.add_argument("--log-s", type=int, default=27)
     ap.add_argument("--passes", type=int, default=2)
-    ap.add_argument("--legs", default="fk,multicol,hash,batched,serving,cpu",
-                    help="comma list of legs to run: fk (the headline line), multicol, hash, batched, serving, cpu")
+    ap.add_argument("--legs", default="fk,multicol,hash,bigbuild,batched,serving,cpu",
+                    help="comma list of legs to run: fk (the headline line), multicol, hash, bigbuild, batched, "
+                         "serving, cpu")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hash", action="store_true", help="skip the hash kernel leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
@@ -728,6 +778,10 @@ def main():
         fk.pop("ds", None), fk.pop("dr", None), fk.pop("out", None)
     mc = None
     torch.cuda.empty_cache()
+    if "bigbuild" in legs:
+        print("bench: 2^26 x 2^27 build leg (wide and standard build+probe blocks)", file=sys.stderr, flush=True)
+        out["bigbuild"] = bigbuild_leg(ctx)
+        torch.cuda.empty_cache()
     if args.calib_copy:  # a plain 1 GiB device copy, three times: the scale of FETCH_SIZE / WRITE_SIZE
         x = torch.empty(CALIB_BYTES // 8, dtype=torch.int64, device="cuda").fill_(1)
         y = torch.empty_like(x)
@@ -771,10 +825,11 @@ def fk_leg(ctx, args, workload_tag, keep_host):
     if args.no_tune:  # planner bits (profiling runs: no extra joins in the trace)
         opts = vjoin.join_opts("fnv", 64, 64, passes=args.passes)
     else:  # radix bits autotuned for the configured pass count (vj_tune, on these tables)
-        (tp, tb1, tb2), tune_ms = ctx.tune(build, probe, vjoin.join_opts("fnv", 64, 64, passes=args.passes), reps=3)
-        opts = vjoin.join_opts("fnv", 64, 64, passes=tp, bits1=tb1, bits2=tb2)
-        print("bench: autotuned plan passes=%d bits=%d+%d (%.3f ms)" % (tp, tb1, tb2, tune_ms), file=sys.stderr,
-              flush=True)
+        (tp, tb1, tb2, tt, ttab), tune_ms = ctx.tune(build, probe, vjoin.join_opts("fnv", 64, 64, passes=args.passes),
+                                                     reps=3)
+        opts = vjoin.join_opts("fnv", 64, 64, passes=tp, bits1=tb1, bits2=tb2, tile=tt, table=ttab)
+        print("bench: autotuned plan passes=%d bits=%d+%d tile=%d table=%d (%.3f ms)" % (tp, tb1, tb2, tt, ttab, tune_ms),
+              file=sys.stderr, flush=True)
     t_alloc = time.perf_counter()
     out_r = torch.empty(ns, dtype=torch.int32, device="cuda")
     out_s = torch.empty(ns, dtype=torch.int32, device="cuda")

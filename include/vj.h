@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define VJ_ABI_VERSION 2
+#define VJ_ABI_VERSION 3
 #define VJ_MAX_KEY_COLS 4
 #define VJ_MAX_ROWS 4294901760ull /* 2^32 - 2^16 */
 
@@ -145,7 +145,20 @@ typedef struct vj_join_opts {
     int32_t bits2;      /* radix bits of pass 2 (0 = planner; 1..8) */
     int32_t one_based;  /* emit row ids + 1 (the reference's primary_keys = row + 1) */
     uint64_t out_capacity_hint; /* expected pair count for vj_join (0 = probe rows) */
+    int32_t tile;       /* radix-pass tile: 0 = planner / autotuned, VJ_TILE_STANDARD or VJ_TILE_SMALL */
+    int32_t table;      /* build+probe block: 0 = planner / autotuned, VJ_TABLE_STANDARD or VJ_TABLE_WIDE */
 } vj_join_opts;
+
+/* Kernel shapes (vj_join_opts.tile / .table). All joins of one pipeline (a
+ * batch) share them. The planner takes the standard tile, and the wide table
+ * when the standard one would need more than 14 radix bits (build sides over
+ * 2^24 rows of 8-byte keys); vj_tune times the alternatives. */
+#define VJ_TILE_STANDARD 1   /* 512 threads x 6 rows (keys <= 8 bytes), x 4 (<= 16), x 2 (wider) */
+#define VJ_TILE_SMALL 2      /* 512 threads x 2 rows: more blocks for small tables */
+#define VJ_TABLE_STANDARD 1  /* 512 threads (8 waves), up to four blocks per CU, LDS table of 2048 rows
+                                (keys <= 8 bytes) or 1024 rows */
+#define VJ_TABLE_WIDE 2      /* 1024 threads (16 waves), one block per CU, LDS table of 8192 rows
+                                (keys <= 8 bytes), 4096 (<= 24 bytes) or 2048 */
 
 void vj_join_opts_default(vj_join_opts *opts);
 
@@ -260,6 +273,10 @@ typedef struct vj_join_summary {
     uint64_t bytes_histogram, bytes_pass1, bytes_pass2, bytes_build_probe, bytes_total;
     double gbps_histogram, gbps_pass1, gbps_pass2, gbps_build_probe, gbps_total;
     double probe_tuples_per_s; /* probe rows / ms_total */
+    int32_t tile_rows;         /* rows per radix-pass tile */
+    int32_t probe_threads;     /* threads per build+probe block */
+    int32_t table_rows;        /* rows per build chunk (LDS table capacity) */
+    int32_t reserved2;
 } vj_join_summary;
 
 vj_status vj_join_summary_get(vj_ctx *ctx, vj_join_summary *summary);
@@ -298,15 +315,20 @@ vj_status vj_join_batch(vj_ctx *ctx, int32_t n_joins, const vj_table *builds, co
 
 /*
  * Autotuner (no counterpart in the reference, whose radix width per level is
- * a configuration value, join_provider.cuh:296-300). Times every candidate
- * plan near the planner's choice (each pass count; radix bits -1..+1; both
- * splits of two passes) on the caller's tables, `reps` timed runs each after
- * a warm-up, and keeps the fastest in the context for the size class of
- * `build` (build rows rounded up to a power of two, key bytes, hash path).
- * Later joins on this context with opts->passes == -1 and no bits set use the
- * tuned plan for their size class (vj_join, vj_join_into, vj_join_batch).
- * With opts->passes >= 0 only plans of that pass count are tried.
- * *best receives opts with the chosen passes/bits, *best_ms its median time.
+ * a configuration value, join_provider.cuh:296-300, as are its threads and
+ * rows per thread, ProbeConfig, join_provider.cu:177-192). Times every candidate plan near the
+ * planner's choice for each table shape (each pass count; radix bits -1..+1;
+ * both splits of two passes; standard and wide build+probe blocks), then the
+ * small pass tile with the fastest, on the caller's tables, `reps` timed runs
+ * each after a warm-up, and keeps the fastest in the context for the size
+ * class of `build` (build rows rounded up to a power of two, key bytes, hash
+ * path). Later joins on this context with opts->passes == -1 and no bits set
+ * use the tuned plan for their size class (vj_join, vj_join_into,
+ * vj_join_batch; a batch takes its shapes from its largest build's class).
+ * With opts->passes >= 0 only plans of that pass count are tried; a nonzero
+ * opts->tile or opts->table fixes that shape.
+ * *best receives opts with the chosen passes/bits/tile/table, *best_ms its
+ * median time.
  * Allocates a temporary output; synchronous.
  */
 vj_status vj_tune(vj_ctx *ctx, const vj_table *build, const vj_table *probe, const vj_join_opts *opts, int32_t reps,

@@ -95,3 +95,30 @@ def test_row_limit_matches_header():
     text = open(HEADER).read()
     m = re.search(r"#define VJ_MAX_ROWS (\d+)ull", text)
     assert m and int(m.group(1)) == (1 << 32) - (1 << 16)
+
+
+# ctypes mirrors in vjoin.py and the C structs they stand for
+MIRRORS = {"Table": "vj_table", "HashOpts": "vj_hash_opts", "JoinOpts": "vj_join_opts", "Rows": "vj_rows",
+           "Pairs": "vj_pairs", "Request": "vj_request", "Response": "vj_response",
+           "ServerStats": "vj_server_stats", "JoinSummary": "vj_join_summary"}
+
+
+def test_ctypes_mirrors_match_the_header(tmp_path):
+    """Every ctypes mirror has the size and field offsets of its C struct, as gcc lays it out."""
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "vj.h"', "int main(void) {"]
+    want = []
+    for py, c in MIRRORS.items():
+        cls = getattr(vjoin, py)
+        lines.append('printf("%%zu\\n", sizeof(%s));' % c)
+        want.append(ctypes.sizeof(cls))
+        for name, _ in cls._fields_:
+            lines.append('printf("%%zu\\n", offsetof(%s, %s));' % (c, name))
+            want.append(getattr(cls, name).offset)
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == want

@@ -139,6 +139,85 @@ def test_zipf_multicolumn_two_passes_mid_size(ctx, kind):
     _assert_same(got, want)
 
 
+# (key width, key columns, hash kind, passes, bits1, bits2, build rows): cases where a partition is larger
+# than a standard table (several build chunks) but fits a wide one, and small ones
+SHAPE_CASES = [(4, 1, "fnv", 1, 3, 0, 40000), (8, 1, "fnv", 2, 3, 2, 150000), (4, 4, "custom_xor_hw", 2, 2, 2, 20000),
+               (8, 2, "fnv", 1, 4, 0, 50000), (8, 1, "custom_add", 1, 2, 0, 30000), (8, 3, "fnv", 0, 0, 0, 3000),
+               (4, 2, "fnv", -1, 0, 0, 70000)]
+
+
+@pytest.mark.parametrize("tile,table", [(1, 1), (2, 1), (1, 2), (2, 2)])
+@pytest.mark.parametrize("case", SHAPE_CASES)
+def test_kernel_shapes(ctx, tile, table, case):
+    """Every pass tile and build+probe block shape gives the same pair set as the
+    oracle (duplicate keys, misses, partitions over a standard table's rows)."""
+    kw, ncols, kind, passes, b1, b2, nr = case
+    rng = np.random.default_rng(nr + ncols * 7 + kw)
+    dt = np.uint32 if kw == 4 else np.uint64
+    hi = 2**31 if kw == 4 else 2**60
+    distinct = rng.integers(0, hi, size=(nr // 2, ncols), dtype=np.uint64).astype(dt)
+    r = np.repeat(distinct, 2, axis=0)[rng.permutation(2 * (nr // 2))]  # every key twice
+    s = distinct[rng.integers(0, len(distinct), 3 * nr)]
+    s[::4, 0] ^= dt(1 << 20)  # some misses
+    rc, sc = [r[:, c].copy() for c in range(ncols)], [s[:, c].copy() for c in range(ncols)]
+    got = _pairs(*_run(ctx, rc, sc, kind=kind, passes=passes, bits1=b1, bits2=b2, tile=tile, table=table))
+    sm = ctx.summary()
+    kb = kw * ncols
+    assert sm["tile_rows"] == 512 * (2 if tile == 2 else (6 if kb <= 8 else (4 if kb <= 16 else 2)))
+    assert sm["probe_threads"] == (1024 if table == 2 else 512)
+    assert sm["table_rows"] == ({4: 8192, 8: 8192, 16: 4096, 24: 4096}[kb] if table == 2 else (2048 if kb <= 8 else 1024))
+    want = sorted_pairs(*oracle.join(workloads.reference_rows(rc), workloads.reference_rows(sc), kind, 64, 64,
+                                     passes=max(passes, 0) if passes >= 0 else 0, bits1=b1, bits2=b2)) \
+        if passes >= 0 else np_join(workloads.reference_rows(rc), workloads.reference_rows(sc))
+    _assert_same(got, want, (case, tile, table))
+
+
+def test_wide_table_heavy_duplicates(ctx):
+    # one key far beyond a wide table's rows: the build side is still cut into chunks
+    r = np.full(40000, 42, dtype=np.uint32)
+    r[::3] = 7
+    s = np.array([42, 7, 1, 42] * 100, dtype=np.uint32)
+    got = _pairs(*_run(ctx, [r], [s], passes=1, bits1=2, table=2))
+    want = np_join(r.reshape(-1, 1).astype(np.uint64), s.reshape(-1, 1).astype(np.uint64))
+    assert len(want[0]) > 3_000_000
+    _assert_same(got, want)
+
+
+def test_shapes_in_batches_and_resident_tables(ctx):
+    import torch
+    joins = workloads.batched(n_joins=12, log_min=8, log_max=16, seed=21)
+    builds, probes, outs_r, outs_s, caps, keep = _batch_inputs(joins)
+    want = [np_join(workloads.reference_rows(r), workloads.reference_rows(s)) for r, s in joins]
+    for tile, table in ((2, 2), (2, 1), (1, 2)):
+        rc, counts = ctx.join_batch(builds, probes, vjoin.join_opts(tile=tile, table=table), outs_r, outs_s, caps)
+        assert rc == 0, ctx.last_error()
+        for i in range(len(joins)):
+            _assert_same(_pairs(outs_r[i][:counts[i]], outs_s[i][:counts[i]]), want[i], (tile, table, i))
+    # a resident build side probed through a wide table
+    rcols, scols = joins[-1]
+    opts = vjoin.join_opts(table=2)
+    built = ctx.build(builds[-1], opts)
+    try:
+        cap = len(scols[0])
+        o_r = torch.empty(cap, dtype=torch.int32, device="cuda")
+        o_s = torch.empty(cap, dtype=torch.int32, device="cuda")
+        rc, n = ctx.probe_into(built, probes[-1], opts, o_r, o_s)
+        assert rc == 0, ctx.last_error()
+        assert ctx.summary()["table_rows"] == 2048 * 4
+        _assert_same(_pairs(o_r[:n], o_s[:n]), want[-1], "resident")
+    finally:
+        built.free()
+
+
+def test_bad_shapes_are_rejected(ctx):
+    import torch
+    d = torch.arange(100, dtype=torch.int32, device="cuda")
+    for tile, table in ((3, 0), (0, 3), (-1, 0)):
+        out = torch.empty(100, dtype=torch.int32, device="cuda")
+        rc, _ = ctx.join_into(vjoin.table([d]), vjoin.table([d]), vjoin.join_opts(tile=tile, table=table), out, out)
+        assert rc == vjoin.STATUS_CODE["VJ_ERR_INVALID_ARGUMENT"], (tile, table, rc)
+
+
 @pytest.mark.parametrize("nr,ns", [(0, 100), (100, 0), (0, 0), (1, 1), (5, 70000)])
 def test_empty_and_tiny(ctx, nr, ns):
     import torch
@@ -327,12 +406,16 @@ def test_tune_keeps_fastest_plan_and_results():
     c = vjoin.Context(0)
     try:
         c.set_profiling(True)
-        (P, b1, b2), ms = c.tune(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts(), reps=2)
+        (P, b1, b2, tile, table), ms = c.tune(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts(), reps=2)
         assert ms > 0 and P in (0, 1, 2)
         assert (P == 0 and b1 == 0) or (P >= 1 and 1 <= b1 <= 8) and (P < 2 or 1 <= b2 <= 8)
+        assert tile in (vjoin.TILE_STANDARD, vjoin.TILE_SMALL) and table in (vjoin.TABLE_STANDARD, vjoin.TABLE_WIDE)
         r, s_ = c.join(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts())
         sm = c.summary()
         assert (sm["passes"], sm["bits1"], sm["bits2"] if P == 2 else 0) == (P, b1, b2 if P == 2 else 0)
+        # the tuned kernel shapes are used too (4-byte keys: tiles of 3072 or 1024 rows, tables of 2048 or 8192)
+        assert sm["tile_rows"] == (3072 if tile == vjoin.TILE_STANDARD else 1024)
+        assert (sm["probe_threads"], sm["table_rows"]) == ((512, 2048) if table == vjoin.TABLE_STANDARD else (1024, 8192))
         _assert_same(_pairs(r, s_), want, "tuned")
         c.tune_clear()
         r, s_ = c.join(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts())

@@ -9,7 +9,9 @@
     jobs split their histogram segments over several blocks; every join against
     an independent numpy join;
   * resident 2-pass build sides re-probed by 2^18..2^22-row probes, directly and
-    through the serving engine mixed with fresh joins, against the numpy join.
+    through the serving engine mixed with fresh joins, against the numpy join;
+  * a 2^26 x 2^27 FK join of int64 keys (past the standard plan's 14 bits) with
+    the wide build+probe block and with the standard one, against the oracle.
 
 Pairs are compared as sorted (build_rowid << 32 | probe_rowid) keys; the sort
 runs on the device (torch.sort) only to compare, the pairs come from libvj.so
@@ -87,6 +89,40 @@ def test_multicol_skew_full_size_matches_oracle(ctx, multicol_full, kind):
                          cap=m)
     assert len(wr) == m
     _assert_same(_dev_keys(out_r, out_s, cnt), _host_keys(wr, ws), kind)
+
+
+# --- a build side past the 14-bit two-pass plan: 2^26 x 2^27 FK join ------------------
+
+
+def test_build_of_2_26_rows_fits_wide_tables_and_matches_oracle(ctx):
+    """A 2^26-row build of 8-byte keys needs 16 radix bits to reach half a
+    standard (2048-row) table per partition, past the 14 bits of two passes.
+    The planner then takes the wide build+probe block (8192-row tables), so 14
+    bits leave one build chunk per partition; the forced standard block still
+    gives the same pairs by cutting partitions into several build chunks."""
+    import torch
+    rcols, scols = workloads.large_fk(log_r=26, log_s=27, seed=26)
+    nr, ns = len(rcols[0]), len(scols[0])
+    dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+    out_r, out_s = _out(ns)
+    rc, cnt = ctx.join_into(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts("fnv", 64, 64), out_r, out_s)
+    assert rc == 0, ctx.last_error()
+    sm = ctx.summary()
+    assert cnt == ns and (sm["passes"], sm["bits1"] + sm["bits2"]) == (2, 14)
+    assert (sm["probe_threads"], sm["table_rows"]) == (1024, 8192)
+    assert sm["tasks"] == 1 << 14  # one build chunk and one probe chunk (of 16384 rows) per partition
+    got = _dev_keys(out_r, out_s, cnt)
+    wr, ws = oracle.join(rcols[0].view(np.uint64).reshape(-1, 1), scols[0].view(np.uint64).reshape(-1, 1), "fnv",
+                         64, 64, passes=2, bits1=sm["bits1"], bits2=sm["bits2"], cap=ns)
+    assert len(wr) == ns
+    _assert_same(got, _host_keys(wr, ws), "wide")
+    del wr, ws
+    rc, cnt = ctx.join_into(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts("fnv", 64, 64, table=1), out_r, out_s)
+    assert rc == 0, ctx.last_error()
+    assert ctx.summary()["table_rows"] == 2048 and ctx.summary()["tasks"] > 1 << 14
+    _assert_same(_dev_keys(out_r, out_s, cnt), got, "standard")
+    del out_r, out_s, dr, ds
+    torch.cuda.empty_cache()
 
 
 # --- batches of uneven joins at real sizes --------------------------------------

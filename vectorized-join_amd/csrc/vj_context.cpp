@@ -56,7 +56,7 @@ struct vj_ctx {
     std::vector<uint64_t> last_sig;  // signature of the last eager pipeline (automatic capture)
     bool graph_failed = false;       // capture failed once: stay eager
     uint64_t graph_replays = 0, graph_captures = 0;
-    // Autotuned radix plans (vj_tune): size class -> passes | bits1 << 8 | bits2 << 16
+    // Autotuned plans (vj_tune): size class -> tuned_word(passes, bits1, bits2, tile, table)
     std::unordered_map<uint64_t, uint32_t> tuned;
     // vj_join_into_noalloc: this device's fixed scratch (looked up at creation)
     // and a pinned staging block for its descriptors and counts (allocated at
@@ -153,12 +153,19 @@ int ceil_log2(uint64_t v) {
     return b;
 }
 
+// Radix bits that bring nr build rows to half a build chunk of `cap` rows
+// per partition.
+int need_bits(uint64_t nr, int cap) {
+    const uint64_t target = (uint64_t)cap / 2;
+    return nr > target ? ceil_log2((nr + target - 1) / target) : 0;
+}
+
 // Radix passes and bits for one join (the reference fixes one radix width
 // per level, join_provider.cuh:296-300, and recurses until a partition fits,
-// :569-571; here the partition target is half an LDS build chunk).
-Err plan_bits(uint64_t nr, int kb, const vj_join_opts *o, int *P, int *b1, int *b2) {
-    const uint64_t target = build_cap(kb) / 2;
-    const int need = nr > target ? ceil_log2((nr + target - 1) / target) : 0;
+// :569-571; here the partition target is half an LDS build chunk of the
+// pipeline's table shape).
+Err plan_bits(uint64_t nr, int kb, int table, const vj_join_opts *o, int *P, int *b1, int *b2) {
+    const int need = need_bits(nr, probe_cap(kb, table));
     int passes = o->passes;
     if (passes < -1 || passes > 2) return Err::make(VJ_ERR_INVALID_ARGUMENT, "passes must be -1, 0, 1 or 2");
     if (passes == -1) passes = need == 0 ? 0 : (need <= kMaxPassBits ? 1 : 2);
@@ -247,7 +254,7 @@ void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0
     jd.b1 = P >= 1 ? b1 : 0;
     jd.b2 = P >= 2 ? b2 : 0;
     jd.shift0 = shift0;
-    const uint32_t T = pass_tile(pl.kb);
+    const uint32_t T = pass_tile(pl.kb, pl.shape.tile);
     const uint32_t n1 = 1u << jd.b1, nb = 1u << (jd.b1 + jd.b2);
     jd.base_off = (uint32_t)pl.base_words;
     pl.base_words += (jd.b1 ? nb : 1) + 1;
@@ -351,6 +358,41 @@ uint64_t tune_key(uint64_t nr, int kb, int hp) {
     return (uint64_t)ceil_log2(std::max<uint64_t>(nr, 1)) | (uint64_t)kb << 8 | (uint64_t)hp << 16;
 }
 
+// Tuned plan word: passes | bits1 << 8 | bits2 << 16 | tile << 24 | table << 28.
+uint32_t tuned_word(int P, int b1, int b2, int tile, int table) {
+    return (uint32_t)P | (uint32_t)b1 << 8 | (uint32_t)b2 << 16 | (uint32_t)tile << 24 | (uint32_t)table << 28;
+}
+
+Err check_shapes(const vj_join_opts *o) {
+    if (o->tile < 0 || o->tile > VJ_TILE_SMALL || o->table < 0 || o->table > VJ_TABLE_WIDE)
+        return Err::make(VJ_ERR_INVALID_ARGUMENT, "tile and table must be 0 (planner), 1 (standard) or 2");
+    return Err::ok();
+}
+
+// Kernel shapes of a pipeline (one for all of its joins): the caller's, else
+// the autotuned ones of the size class of the batch's largest build, else
+// the standard tile, and the wide table when a standard one would need more
+// than kMaxHistBits bits of partitioning to bring the largest build down to
+// half a chunk per partition.
+Err choose_shapes(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_opts *o, const vj_ctx *c) {
+    VJ_TRY(check_shapes(o));
+    uint64_t nr = 0;
+    for (const JoinSpec &js : specs) nr = std::max<uint64_t>(nr, js.res ? js.res->n : js.r->n_rows);
+    int tile = o->tile - 1, table = o->table - 1;  // -1: choose
+    if (c && o->passes == -1 && !o->bits1 && !o->bits2 && !c->tuned.empty()) {
+        auto it = c->tuned.find(tune_key(nr, pl.kb, pl.shape.hp));
+        if (it != c->tuned.end()) {
+            if (tile < 0) tile = (int)((it->second >> 24) & 0xF);
+            if (table < 0) table = (int)((it->second >> 28) & 0xF);
+        }
+    }
+    if (tile < 0) tile = 0;
+    if (table < 0) table = need_bits(nr, build_cap(pl.kb)) > kMaxHistBits ? 1 : 0;
+    pl.shape.tile = tile;
+    pl.shape.table = table;
+    return Err::ok();
+}
+
 bool same_hash(const vj_hash_opts &a, const vj_hash_opts &b) {
     return a.kind == b.kind && a.hash_bits == b.hash_bits && a.chunk_bits == b.chunk_bits && a.word_mode == b.word_mode;
 }
@@ -364,6 +406,9 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
     pl.shape.hp = hash_path(pl.hash);
     pl.shape.kw = t0->key_width;
     pl.shape.nc = t0->n_cols;
+    for (const JoinSpec &js : specs)
+        if (!js.res) VJ_TRY(check_table(js.r));
+    VJ_TRY(choose_shapes(pl, specs, o, c));
     for (const JoinSpec &js : specs) pl.hist_rows_total += (js.res ? 0 : js.r->n_rows) + js.s->n_rows;
     pl.hist_blocks_target = 2u * (uint32_t)(c ? c->cus : 256);  // sizing without a context: an MI355X
     for (size_t i = 0; i < specs.size(); i++) {
@@ -385,7 +430,7 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
             jd.job_r = add_resident_job(pl, b);
         } else {
             VJ_TRY(check_pair(js.r, js.s));
-            VJ_TRY(plan_bits(js.r->n_rows, pl.kb, o, &P, &b1, &b2));
+            VJ_TRY(plan_bits(js.r->n_rows, pl.kb, pl.shape.table, o, &P, &b1, &b2));
             if (c && o->passes == -1 && !o->bits1 && !o->bits2 && !c->tuned.empty()) {  // autotuned plan
                 auto it = c->tuned.find(tune_key(js.r->n_rows, pl.kb, pl.shape.hp));
                 if (it != c->tuned.end()) {
@@ -422,7 +467,7 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
         jd.out_r = js.out_r;
         jd.out_s = js.out_s;
         jd.cap = js.cap;
-        jd.cap_r = build_cap(pl.kb);
+        jd.cap_r = probe_cap(pl.kb, pl.shape.table);
         jd.cap_s = kProbeChunk;
         jd.nparts = 1u << (P >= 1 ? b1 : 0) << (P >= 2 ? b2 : 0);
         jd.ptask_off = (uint32_t)pl.ptask_words;
@@ -552,7 +597,7 @@ PipeArgs make_args(const Arenas &ar, const Plan &pl) {
     a.total_p1_tiles = pl.total_p1_tiles;
     a.total_p2_chunks = pl.total_p2_chunks;
     a.max_hist_bins = pl.max_hist_bins;
-    a.pass_tile = pass_tile(pl.kb);
+    a.pass_tile = pass_tile(pl.kb, pl.shape.tile);
     a.hash = pl.hash;
     return a;
 }
@@ -607,7 +652,8 @@ std::vector<uint64_t> graph_signature(const vj_ctx *c, const Plan &pl, const Pip
     memcpy(sig.data(), &a, sizeof(PipeArgs));
     uint64_t *x = sig.data() + (sizeof(PipeArgs) + 7) / 8;
     x[0] = (uint64_t)pl.any_part | (uint64_t)pl.any_p2 << 1;
-    x[1] = (uint64_t)pl.shape.hp | (uint64_t)pl.shape.kw << 8 | (uint64_t)pl.shape.nc << 16;
+    x[1] = (uint64_t)pl.shape.hp | (uint64_t)pl.shape.kw << 8 | (uint64_t)pl.shape.nc << 16 |
+           (uint64_t)pl.shape.tile << 24 | (uint64_t)pl.shape.table << 32;
     x[2] = (uint64_t)c->cus * (uint64_t)probe_blocks_per_cu(pl.shape);
     x[3] = pl.upload_bytes;
     x[4] = pl.off_count;
@@ -714,6 +760,9 @@ void fill_summary(vj_ctx *c, const Plan &pl, const JoinSpec &js, uint64_t matche
     }
     s.scratch_bytes = scratch_bytes;
     s.graph_replay = replay ? 1 : 0;
+    s.tile_rows = pass_tile(pl.kb, pl.shape.tile);
+    s.probe_threads = probe_threads(pl.shape.table);
+    s.table_rows = probe_cap(pl.kb, pl.shape.table);
     stage_bytes(pl, matches, s);
     s.gbps_histogram = gbps(s.bytes_histogram, s.ms_histogram);
     s.gbps_pass1 = gbps(s.bytes_pass1, s.ms_pass1);
@@ -988,6 +1037,8 @@ void vj_join_opts_default(vj_join_opts *o) {
     o->bits2 = 0;
     o->one_based = 0;
     o->out_capacity_hint = 0;
+    o->tile = 0;
+    o->table = 0;
 }
 
 vj_status vj_join_into(vj_ctx *c, const vj_table *build, const vj_table *probe, const vj_join_opts *opts,
@@ -1283,7 +1334,12 @@ vj_status vj_build(vj_ctx *c, const vj_table *build, const vj_join_opts *opts, v
         pl.shape.hp = hash_path(pl.hash);
         pl.shape.kw = build->key_width;
         pl.shape.nc = build->n_cols;
-        e = plan_bits(build->n_rows, pl.kb, opts, &P, &b1, &b2);
+        e = check_shapes(opts);
+        if (!e.failed()) {
+            pl.shape.tile = opts->tile ? opts->tile - 1 : 0;
+            pl.shape.table = opts->table ? opts->table - 1 : (need_bits(build->n_rows, build_cap(pl.kb)) > kMaxHistBits ? 1 : 0);
+            e = plan_bits(build->n_rows, pl.kb, pl.shape.table, opts, &P, &b1, &b2);
+        }
     }
     if (!e.failed()) {
         if (P == 0) {
@@ -1377,38 +1433,51 @@ vj_status vj_tune(vj_ctx *c, const vj_table *build, const vj_table *probe, const
     HashCfg hc;
     e = make_hash_cfg(&opts->hash, build, &hc);
     if (e.failed()) return fail(c, e);
+    e = check_shapes(opts);
+    if (e.failed()) return fail(c, e);
     const int kb = build->key_width * build->n_cols, hp = hash_path(hc);
     const uint64_t nr = build->n_rows;
-    // candidates around the planner's choice: every pass count, bits -1..+1
-    const uint64_t target = build_cap(kb) / 2;
-    const int need = nr > target ? ceil_log2((nr + target - 1) / target) : 0;
-    std::vector<std::array<int, 3>> cands;
-    if (nr <= (uint64_t)build_cap(kb)) cands.push_back({0, 0, 0});
-    for (int b = std::max(1, need - 1); b <= std::min(kMaxPassBits, need + 1); b++) cands.push_back({1, b, 0});
-    for (int tot = std::max(2, need - 1); tot <= std::min(kMaxHistBits, need + 1); tot++) {
-        for (int b1 : {(tot + 1) / 2, tot / 2}) {
-            const int b2 = tot - b1;
-            if (b1 < 1 || b1 > kMaxPassBits || b2 < 1 || b2 > kMaxPassBits) continue;
-            bool seen = false;
-            for (auto &x : cands) seen |= x[0] == 2 && x[1] == b1 && x[2] == b2;
-            if (!seen) cands.push_back({2, b1, b2});
-        }
+    // Candidates: for each table shape, plans around the planner's choice for
+    // that table's chunk (every pass count, bits -1..+1, both splits of two
+    // passes), with the standard tile; then the small tile with the fastest.
+    struct Cand {
+        int P, b1, b2, tile, table;
+    };
+    std::vector<Cand> cands;
+    auto add = [&](Cand x) {
+        for (auto &y : cands)
+            if (y.P == x.P && y.b1 == x.b1 && y.b2 == x.b2 && y.tile == x.tile && y.table == x.table) return;
+        if (opts->passes >= 0 && x.P != opts->passes) return;
+        cands.push_back(x);
+    };
+    const int tile0 = opts->tile ? opts->tile - 1 : 0;
+    for (int table = 0; table <= 1; table++) {
+        if (opts->table && table != opts->table - 1) continue;
+        const int cap = probe_cap(kb, table);
+        const int need = need_bits(nr, cap);
+        if (nr <= (uint64_t)cap) add({0, 0, 0, tile0, table});
+        for (int b = std::max(1, need - 1); b <= std::min(kMaxPassBits, need + 1); b++) add({1, b, 0, tile0, table});
+        for (int tot = std::max(2, need - 1); tot <= std::min(kMaxHistBits, need + 1); tot++)
+            for (int b1 : {(tot + 1) / 2, tot / 2}) {
+                const int b2 = tot - b1;
+                if (b1 >= 1 && b1 <= kMaxPassBits && b2 >= 1 && b2 <= kMaxPassBits) add({2, b1, b2, tile0, table});
+            }
     }
-    if (opts->passes >= 0) {  // a fixed pass count: tune the bits only
-        std::vector<std::array<int, 3>> keep;
-        for (auto &x : cands)
-            if (x[0] == opts->passes) keep.push_back(x);
-        if (keep.empty()) return fail(c, Err::make(VJ_ERR_UNSUPPORTED, "no candidate plan with that pass count"));
-        cands.swap(keep);
-    }
+    if (cands.empty()) return fail(c, Err::make(VJ_ERR_UNSUPPORTED, "no candidate plan with that pass count"));
     // output buffers sized by a counting run (the count does not depend on the plan)
     std::vector<JoinSpec> specs{{build, probe, nullptr, nullptr, 0}};
     std::vector<uint64_t> counts;
     c->tuned.erase(tune_key(nr, kb, hp));
-    vj_join_opts oc = *opts;  // counting run: any valid plan
-    oc.passes = cands[0][0];
-    oc.bits1 = cands[0][1];
-    oc.bits2 = cands[0][2];
+    auto plan_opts = [&](const Cand &x) {
+        vj_join_opts o2 = *opts;
+        o2.passes = x.P;
+        o2.bits1 = x.b1;
+        o2.bits2 = x.b2;
+        o2.tile = x.tile + 1;
+        o2.table = x.table + 1;
+        return o2;
+    };
+    vj_join_opts oc = plan_opts(cands[0]);  // counting run: any valid plan
     e = run_joins(c, specs, &oc, s, counts, false, 0, nullptr);
     if (e.failed()) return fail(c, e);
     const uint64_t cap = std::max<uint64_t>(counts[0], 1);
@@ -1421,34 +1490,44 @@ vj_status vj_tune(vj_ctx *c, const vj_table *build, const vj_table *probe, const
     specs[0].out_r = ob;
     specs[0].out_s = op;
     specs[0].cap = cap;
-    double best_t = 1e30;
-    std::array<int, 3> best_c = cands[0];
-    for (auto &cand : cands) {
-        vj_join_opts o2 = *opts;
-        o2.passes = cand[0];
-        o2.bits1 = cand[1];
-        o2.bits2 = cand[2];
+    auto time_plan = [&](const Cand &x, double *med) {
+        const vj_join_opts o2 = plan_opts(x);
         std::vector<double> ts;
-        for (int r = 0; r <= reps && !e.failed(); r++) {  // the first run warms the arenas
-            e = run_joins(c, specs, &o2, s, counts, false, 0, nullptr);
+        for (int r = 0; r <= reps; r++) {  // the first run warms the arenas
+            Err er = run_joins(c, specs, &o2, s, counts, false, 0, nullptr);
+            if (er.failed()) return er;
             if (r) ts.push_back(c->summary.ms_total);
         }
-        if (e.failed()) break;
         std::sort(ts.begin(), ts.end());
-        const double med = ts[ts.size() / 2];
-        if (med < best_t) {
-            best_t = med;
-            best_c = cand;
+        *med = ts[ts.size() / 2];
+        return Err::ok();
+    };
+    double best_t = 1e30;
+    Cand best_c = cands[0];
+    for (const Cand &x : cands) {
+        double t = 0;
+        e = time_plan(x, &t);
+        if (e.failed()) break;
+        if (t < best_t) {
+            best_t = t;
+            best_c = x;
+        }
+    }
+    if (!e.failed() && !opts->tile && best_c.P > 0) {  // the small tile, for the fastest plan
+        Cand x = best_c;
+        x.tile = 1;
+        double t = 0;
+        e = time_plan(x, &t);
+        if (!e.failed() && t < best_t) {
+            best_t = t;
+            best_c = x;
         }
     }
     (void)hipFree(ob);
     (void)hipFree(op);
     if (e.failed()) return fail(c, e);
-    c->tuned[tune_key(nr, kb, hp)] = (uint32_t)best_c[0] | (uint32_t)best_c[1] << 8 | (uint32_t)best_c[2] << 16;
-    *best = *opts;
-    best->passes = best_c[0];
-    best->bits1 = best_c[1];
-    best->bits2 = best_c[2];
+    c->tuned[tune_key(nr, kb, hp)] = tuned_word(best_c.P, best_c.b1, best_c.b2, best_c.tile, best_c.table);
+    *best = plan_opts(best_c);
     if (best_ms) *best_ms = best_t;
     return VJ_OK;
 }

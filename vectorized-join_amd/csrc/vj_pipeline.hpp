@@ -37,22 +37,47 @@ namespace vj {
 constexpr int kPassThreads = 512;   // radix pass kernels
 constexpr int kPassWaves = kPassThreads / 64;
 constexpr int kHistThreads = 1024;  // histogram kernel
-constexpr int kProbeThreads = 512;  // build+probe kernel
-constexpr int kProbeWaves = kProbeThreads / 64;
+constexpr int kMaxProbeThreads = 1024;  // build+probe kernel (512 or 1024 threads, see probe_threads)
+constexpr int kMaxProbeWaves = kMaxProbeThreads / 64;
 constexpr int kMaxSegments = 512;  // segments (histogram blocks) per job
 constexpr int kMaxPassBits = 8;     // digits per pass <= 256
 constexpr int kMaxHistBits = 14;    // 2D histogram in LDS: 2^14 x 4 B = 64 KiB
 constexpr uint32_t kXcds = 8;       // accelerator dies (XCDs) of an MI355X
 constexpr uint32_t kXcdGroup = 32;  // consecutive pass-2 chunks kept on one XCD
 
-// Tuples per pass tile, per build chunk and per probe chunk, by key bytes per row.
-__host__ __device__ constexpr int pass_items(int key_bytes) { return key_bytes <= 8 ? 6 : (key_bytes <= 16 ? 4 : 2); }
-__host__ __device__ constexpr int pass_tile(int key_bytes) { return kPassThreads * pass_items(key_bytes); }
+// Kernel shapes, chosen per pipeline by the planner or the autotuner.
+//  tile  0: standard pass tile, 512 threads x pass_items rows (6 / 4 / 2 rows by
+//           key bytes); 1: small tile, 512 threads x 2 rows (more blocks for
+//           small tables).
+//  table 0: standard build+probe block, 512 threads (up to four blocks per
+//           CU) around an LDS table of build_cap rows; 1: wide block, 1024
+//           threads (one block per CU) around a table of wide_cap rows, so a
+//           partition four to eight times larger fits one build chunk.
+__host__ __device__ constexpr int pass_items(int key_bytes, int tile = 0) {
+    return tile ? 2 : (key_bytes <= 8 ? 6 : (key_bytes <= 16 ? 4 : 2));
+}
+__host__ __device__ constexpr int pass_tile(int key_bytes, int tile = 0) { return kPassThreads * pass_items(key_bytes, tile); }
 __host__ __device__ constexpr int build_cap(int key_bytes) { return key_bytes <= 8 ? 2048 : 1024; }
+// LDS bytes per build row of a table: the key, two 4-byte slots, a 2-byte chain link.
+__host__ __device__ constexpr int table_row_bytes(int key_bytes) { return key_bytes + 10; }
+constexpr int kWideTableBytes = 150 * 1024;  // of the 160 KiB of LDS per CU
+// Largest power of two <= 8192 whose table fits kWideTableBytes (row indexes
+// stay below 2^16 - 1, the no-match marker).
+__host__ __device__ constexpr int wide_cap(int key_bytes) {
+    int c = 8192;
+    while (c > 1024 && c * table_row_bytes(key_bytes) > kWideTableBytes) c >>= 1;
+    return c;
+}
+__host__ __device__ constexpr int probe_cap(int key_bytes, int table) { return table ? wide_cap(key_bytes) : build_cap(key_bytes); }
+__host__ __device__ constexpr int probe_threads(int table) { return table ? 1024 : 512; }
 constexpr int kProbeChunk = 16384;  // probe rows per build+probe task
 static_assert((kProbeChunk & (kProbeChunk - 1)) == 0 && (build_cap(4) & (build_cap(4) - 1)) == 0 &&
-                  (build_cap(8) & (build_cap(8) - 1)) == 0 && (build_cap(16) & (build_cap(16) - 1)) == 0,
+                  (build_cap(8) & (build_cap(8) - 1)) == 0 && (build_cap(16) & (build_cap(16) - 1)) == 0 &&
+                  (wide_cap(8) & (wide_cap(8) - 1)) == 0 && (wide_cap(12) & (wide_cap(12) - 1)) == 0,
               "chunk capacities are powers of two (the task counts are shifts)");
+static_assert(wide_cap(4) == 8192 && wide_cap(8) == 8192 && wide_cap(16) == 4096 && wide_cap(32) == 2048 &&
+                  wide_cap(32) * table_row_bytes(32) <= kWideTableBytes,
+              "wide tables");
 
 struct JobDesc {
     TView in;                 // the caller's table
@@ -130,7 +155,7 @@ struct PipeArgs {
     uint32_t n_jobs, n_joins, n_colscan;
     uint32_t total_hist_blocks, total_p1_tiles, total_p2_chunks;
     uint32_t max_hist_bins;         // LDS bins of the histogram kernel
-    uint32_t pass_tile;             // tuples per radix-pass tile (pass_tile(key bytes))
+    uint32_t pass_tile;             // tuples per radix-pass tile (pass_tile(key bytes, tile shape))
     HashCfg hash;
 };
 
@@ -138,7 +163,21 @@ struct PipeArgs {
 struct LaunchShape {
     int hp, kw, nc;  // hash path, key width, key columns
     int cus;         // compute units of the device
+    int tile, table; // kernel shapes (see pass_items / probe_cap)
 };
+
+// The hashing stages (histogram, radix pass, build+probe), compiled once per
+// hash path in its own translation unit (vj_join_fnv.hip, vj_join_xorhw.hip,
+// vj_join_generic.hip) from vj_join_stages.hpp.
+template <int HP> struct StageLaunch {
+    static hipError_t hist(const PipeArgs &a, const LaunchShape &sh, hipStream_t s);
+    static hipError_t pass(const PipeArgs &a, const LaunchShape &sh, int pass, hipStream_t s);
+    static hipError_t probe(const PipeArgs &a, const LaunchShape &sh, int blocks, hipStream_t s);
+    static int probe_blocks_per_cu(const LaunchShape &sh);
+};
+extern template struct StageLaunch<HP_FNV>;
+extern template struct StageLaunch<HP_XOR_HW>;
+extern template struct StageLaunch<HP_GENERIC>;
 hipError_t launch_hist(const PipeArgs &a, const LaunchShape &sh, hipStream_t s);
 hipError_t launch_scan(const PipeArgs &a, hipStream_t s);
 hipError_t launch_reset_probe(const PipeArgs &a, hipStream_t s);

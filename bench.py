@@ -149,23 +149,22 @@ def _stage_of_dispatches(rows):
     return out
 
 
-def traffic_from_pmc(fetch_csv, write_csv, workload):
+def traffic_from_pmc(fetch_csv, write_csv, workload, leg_args="--legs fk --steps 1 --warmup 1 --no-tune"):
     """HBM bytes per launch of each stage's kernel from two rocprofv3 --pmc runs,
     each counter scaled by what it reads for a plain 1 GiB device copy."""
     import numpy as np
     lines = ["# HBM traffic per launch, from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch),",
              "# each scaled by counter(1 GiB device copy) / 1 GiB. Workload: " + workload,
-             "# made by: rocprofv3 --pmc FETCH_SIZE -d F -o run -- python3 bench.py --legs fk --steps 1 --warmup 1 "
-             "--no-tune --calib-copy",
-             "#          rocprofv3 --pmc WRITE_SIZE -d W -o run -- python3 bench.py --legs fk --steps 1 --warmup 1 "
-             "--no-tune --calib-copy",
+             "# made by: rocprofv3 --pmc FETCH_SIZE -d F -o run -- python3 bench.py %s --calib-copy" % leg_args,
+             "#          rocprofv3 --pmc WRITE_SIZE -d W -o run -- python3 bench.py %s --calib-copy" % leg_args,
              "#          python3 bench.py --traffic-from F/run_results.db W/run_results.db"]
     factors, per = {}, {}
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         rows = _pmc_rows(path, counter)
+        # the calibration copies are the largest copy dispatches of the run
         copies = [v for _, n, v in rows if "copy" in n.lower()]
         big = max(copies)
-        calib = float(np.median([v for v in copies if v > 0.5 * big]))
+        calib = float(np.median([v for v in copies if v >= 0.98 * big]))
         factors[counter] = CALIB_BYTES / calib
         for st, v in _stage_of_dispatches(rows):
             per.setdefault((st, counter), []).append(v)
@@ -177,6 +176,50 @@ def traffic_from_pmc(fetch_csv, write_csv, workload):
         wb = float(np.median(per[(st, "WRITE_SIZE")])) * factors["WRITE_SIZE"]
         lines.append("stage=%s kernel=%s fetch_bytes=%.0f write_bytes=%.0f traffic_bytes=%.0f dispatches=%d" %
                      (st, STAGE_KERNEL[st], fb, wb, fb + wb, len(per[(st, "FETCH_SIZE")])))
+    return "\n".join(lines) + "\n"
+
+
+def counters_from_pmc(db_paths, rows, command, workload):
+    """Per kernel (the two k_pass launches of a join split into pass 1 and pass
+    2), the mean per dispatch of every counter in the given rocprofv3 --pmc
+    databases, and per tuple: instruction counts x 64 / tuples per dispatch
+    (instructions per lane and row), and LDS bank-conflict cycles per LDS
+    instruction cycle."""
+    import collections
+    import re
+    import sqlite3
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    for path in db_paths:
+        db = sqlite3.connect(path)
+        got = db.execute("select dispatch_id, kernel_name, counter_name, value from counters_collection").fetchall()
+        db.close()
+        names = {}
+        for d, n, _, _ in got:
+            names[d] = n
+        label, npass = {}, collections.Counter()
+        for d in sorted(names):
+            m = re.search(r"vj::(?:\(anonymous namespace\)::)?(k_[a-z0-9_]+(<[^>]*>)?)", names[d])
+            short = m.group(1) if m else names[d][:40]
+            if short.startswith("k_pass"):
+                npass[short] += 1
+                short += " pass%d" % (2 - npass[short] % 2)
+            label[d] = short
+        for d, _, c, v in got:
+            if label[d].startswith("k_"):
+                per[label[d]][c].append(float(v))
+    lines = ["# SQ counters per dispatch (mean) and per tuple. Workload: " + workload,
+             "# tuples per dispatch: %d (build + probe rows)" % rows,
+             "# made by: " + command,
+             "# per-row columns: instructions x 64 lanes / tuples; bank = SQ_LDS_BANK_CONFLICT / SQ_ACTIVE_INST_LDS"]
+    for k in sorted(per):
+        c = {n: sum(v) / len(v) for n, v in per[k].items()}
+        parts = ["%s=%.4g" % (n, c[n]) for n in sorted(c)]
+        for n in ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR"):
+            if n in c:
+                parts.append("%s_per_row=%.2f" % (n[9:], c[n] * 64.0 / rows))
+        if c.get("SQ_ACTIVE_INST_LDS"):
+            parts.append("bank=%.2f" % (c.get("SQ_LDS_BANK_CONFLICT", 0.0) / c["SQ_ACTIVE_INST_LDS"]))
+        lines.append("%-28s %s" % (k, " ".join(parts)))
     return "\n".join(lines) + "\n"
 
 
@@ -721,6 +764,14 @@ def main():
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="no GPU: write the traffic summary from two rocprofv3 --pmc results (run_results.db or "
                          "counter_collection.csv)")
+    ap.add_argument("--counters-from", nargs="+", metavar="RUN_RESULTS_DB",
+                    help="summarize rocprofv3 --pmc SQ counter runs (per kernel, per tuple) and exit")
+    ap.add_argument("--counters-rows", type=int, default=0, help="tuples per dispatch (for --counters-from)")
+    ap.add_argument("--counters-out", default="", help="output file (for --counters-from)")
+    ap.add_argument("--traffic-workload", default="", help="workload named in the --traffic-from summary")
+    ap.add_argument("--traffic-out", default="", help="output file of --traffic-from (default: the FK summary)")
+    ap.add_argument("--traffic-leg", default="--legs fk --steps 1 --warmup 1 --no-tune",
+                    help="the bench arguments the --traffic-from runs used")
     ap.add_argument("--trace-summary", nargs=2, metavar=("RUN_RESULTS_DB", "OUT_MD"),
                     help="no GPU: write a markdown per-kernel summary of a rocprofv3 --kernel-trace database")
     ap.add_argument("--trace-command", default="", help="the command that made the trace (for --trace-summary)")
@@ -734,9 +785,20 @@ def main():
         return
     workload_tag = "large_fk build 2^%d probe 2^%d passes %d" % (args.log_r, args.log_s, args.passes)
     if args.traffic_from:
-        text = traffic_from_pmc(args.traffic_from[0], args.traffic_from[1], workload_tag)
-        with open(TRAFFIC_FILE, "w") as f:
+        text = traffic_from_pmc(args.traffic_from[0], args.traffic_from[1], args.traffic_workload or workload_tag,
+                                args.traffic_leg)
+        with open(args.traffic_out or TRAFFIC_FILE, "w") as f:
             f.write(text)
+        print(text, end="")
+        return
+    if args.counters_from:
+        cmd = ("rocprofv3 --pmc <counters> -d D -o run -- python3 bench.py %s, once per counter set; "
+               "python3 bench.py --counters-from D/run_results.db ... --counters-rows %d" % (args.traffic_leg,
+                                                                                         args.counters_rows))
+        text = counters_from_pmc(args.counters_from, args.counters_rows, cmd, args.traffic_workload or workload_tag)
+        if args.counters_out:
+            with open(args.counters_out, "w") as f:
+                f.write(text)
         print(text, end="")
         return
     if args.gpus != 1:

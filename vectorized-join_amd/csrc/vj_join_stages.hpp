@@ -360,7 +360,9 @@ __device__ __forceinline__ WalkArgs walk_args(const ProbeTask &t) {
 
 // Row id of build row o of the task's chunk, read from the partitioned build
 // side (its rows were just loaded by the build step, so the read is cached).
-__device__ __forceinline__ uint32_t build_id(const WalkArgs &wa, uint32_t o) {
+// (IDS: from the table's copy in LDS, `rid`.)
+template <bool IDS> __device__ __forceinline__ uint32_t build_id(const WalkArgs &wa, const uint32_t *rid, uint32_t o) {
+    if constexpr (IDS) return rid[o];
     return wa.r_ids ? wa.r_ids[wa.r_lo + o] : wa.r_lo + o;
 }
 
@@ -536,8 +538,8 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
 // build chunk has equal keys) the rest of its chain. The probe-side and
 // build-side row ids of a group of G iterations are loaded before any is
 // written, so their round trips overlap.
-template <int G, int PV>
-__device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint16_t *nxt,
+template <int G, int PV, bool IDS>
+__device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint16_t *nxt, const uint32_t *rid,
                                             const uint32_t (&res)[ProbeShape<PV>::RES], Emitter &em) {
     constexpr int U = kProbeUnroll;
     constexpr int TH = ProbeShape<PV>::TH, ITERS = ProbeShape<PV>::ITERS;
@@ -559,7 +561,7 @@ __device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint16_t *
                 const bool in = i < s_end;
                 const uint32_t o = res_get(res, (g + it) * U + u);
                 sid[it * U + u] = wa.s.rowids ? wa.s.rowids[in ? i : wa.s_lo] : i;  // wave-uniform test
-                bid[it * U + u] = build_id(wa, in && o != kNoMatch ? o : 0u);
+                bid[it * U + u] = build_id<IDS>(wa, rid, in && o != kNoMatch ? o : 0u);
             }
         }
 #pragma unroll
@@ -575,7 +577,7 @@ __device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint16_t *
                     while (__ballot(x != 0)) {
                         const bool has = x != 0;
                         const uint32_t r = has ? x - 1 : 0u;
-                        em.put(has, (has ? build_id(wa, r) : 0u) + ob1, sid[it * U + u] + ob1, wa);
+                        em.put(has, (has ? build_id<IDS>(wa, rid, r) : 0u) + ob1, sid[it * U + u] + ob1, wa);
                         x = has ? nxt[r] : 0u;
                     }
                 }
@@ -604,6 +606,8 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
     KT *rk = reinterpret_cast<KT *>(smem);                       // [NC][CAP]
     uint32_t *slots = reinterpret_cast<uint32_t *>(rk + NC * CAP);  // [2 * CAP]
     uint16_t *nxt = reinterpret_cast<uint16_t *>(slots + 2 * CAP);  // [CAP] chain of equal keys (index + 1, 0: end)
+    constexpr bool IDS = table_keeps_ids(KB);
+    uint32_t *rid = reinterpret_cast<uint32_t *>(nxt + CAP);  // [CAP] build rows' ids (IDS only)
     __shared__ ProbeTask t;
     uint32_t res[ProbeShape<PV>::RES];  // matched build row per probe row, two per word
 
@@ -692,10 +696,13 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
             const GView rv = uniform_view(t.r);
             for (uint32_t i0 = 0; i0 < r_n; i0 += RB * TH) {
                 Key<KW, NC> rkey[RB];
+                uint32_t rrow[RB];
 #pragma unroll
                 for (int r = 0; r < RB; r++) {
                     const uint32_t i = i0 + tid + r * TH;
-                    rkey[r] = load_key<KW, NC>(rv, r_lo + (i < r_n ? i : 0u));
+                    const uint32_t row = r_lo + (i < r_n ? i : 0u);
+                    rkey[r] = load_key<KW, NC>(rv, row);
+                    if constexpr (IDS) rrow[r] = rv.rowids ? rv.rowids[row] : row;  // wave-uniform test
                 }
 #pragma unroll
                 for (int r = 0; r < RB; r++) {
@@ -703,6 +710,7 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
                     if (i < r_n) {
 #pragma unroll
                         for (int c = 0; c < NC; c++) rk[c * CAP + i] = rkey[r].c[c];
+                        if constexpr (IDS) rid[i] = rrow[r];
                     }
                 }
             }
@@ -761,7 +769,7 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
             em.pos = t.out_base;
             for (uint32_t k = 0; k < w; k++) em.pos += t.wave_count[k];
             // row-id groups of two iterations where registers allow (single 8-byte key, FNV / XOR)
-            replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 2 : 1, PV>(wa, nxt, res, em);
+            replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 2 : 1, PV, IDS>(wa, nxt, rid, res, em);
         }
         __syncthreads();
     }

@@ -58,9 +58,16 @@ __host__ __device__ constexpr int pass_items(int key_bytes, int tile = 0) {
 }
 __host__ __device__ constexpr int pass_tile(int key_bytes, int tile = 0) { return kPassThreads * pass_items(key_bytes, tile); }
 __host__ __device__ constexpr int build_cap(int key_bytes) { return key_bytes <= 8 ? 2048 : 1024; }
-// LDS bytes per build row of a table: the key, two 4-byte slots, a 2-byte chain link.
-__host__ __device__ constexpr int table_row_bytes(int key_bytes) { return key_bytes + 10; }
-constexpr int kWideTableBytes = 150 * 1024;  // of the 160 KiB of LDS per CU
+// Build rows' ids are kept in LDS next to the table for keys wider than 8
+// bytes (whose tables are small), so duplicate chains are written without a
+// global load per entry; narrower keys read them from the partitioned table.
+__host__ __device__ constexpr bool table_keeps_ids(int key_bytes) { return key_bytes > 8; }
+// LDS bytes per build row of a table: the key, two 4-byte slots, a 2-byte chain
+// link, and the row id where the table keeps it.
+__host__ __device__ constexpr int table_row_bytes(int key_bytes) {
+    return key_bytes + 10 + (table_keeps_ids(key_bytes) ? 4 : 0);
+}
+constexpr int kWideTableBytes = 156 * 1024;  // of the 160 KiB of LDS per CU
 // Largest power of two <= 8192 whose table fits kWideTableBytes (row indexes
 // stay below 2^16 - 1, the no-match marker).
 __host__ __device__ constexpr int wide_cap(int key_bytes) {

@@ -768,8 +768,9 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
         if (wc) {
             em.pos = t.out_base;
             for (uint32_t k = 0; k < w; k++) em.pos += t.wave_count[k];
-            // row-id groups of two iterations where registers allow (single 8-byte key, FNV / XOR)
-            replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 2 : 1, PV, IDS>(wa, nxt, rid, res, em);
+            // row-id groups of two iterations where registers allow (single 8-byte key, FNV / XOR), of
+            // four where the build rows' ids come from LDS (keys wider than 8 bytes)
+            replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 2 : (IDS ? 4 : 1), PV, IDS>(wa, nxt, rid, res, em);
         }
         __syncthreads();
     }

@@ -606,10 +606,12 @@ def multicol_data():
     return workloads.to_device(rcols), workloads.to_device(scols), n_hit
 
 
-def multicol_leg(ctx, steps, data):
+def multicol_leg(ctx, steps, data, tune=True, fixed_plan=None):
     """Multi-column skewed join: 4 x uint32 key columns, build 2^22 rows (2^20
     distinct keys, each 4 times), probe 2^25 rows (80 % Zipf(1.0) over the
-    distinct keys, 20 % outside them), with FNV-1a-64 and custom_xor_hw-64.
+    distinct keys, 20 % outside them), with FNV-1a-64 and custom_xor_hw-64;
+    two radix passes, bits and kernel shapes autotuned per hash (vj_tune on
+    these tables) unless tune is off (then the planner's plan).
     Checked by size-independent properties: the match count is 4 x the hitting
     probe rows, every pair has equal keys in all 4 columns, every probe row
     appears 0 or 4 times. (The tests compare the same join with the oracle.)"""
@@ -623,10 +625,20 @@ def multicol_leg(ctx, steps, data):
     out_r = torch.empty(m_want, dtype=torch.int32, device="cuda")
     out_s = torch.empty(m_want, dtype=torch.int32, device="cuda")
     res = {"workload": "build 2^22 rows of 4 x uint32 keys (2^20 distinct, x4), probe 2^25 rows (80% Zipf(1.0), 20% "
-                       "misses), 2 radix passes", "build_rows": nr, "probe_rows": ns, "matches": m_want}
+                       "misses), 2 radix passes, bits and kernel shapes autotuned" if tune else
+                       "build 2^22 rows of 4 x uint32 keys (2^20 distinct, x4), probe 2^25 rows (80% Zipf(1.0), 20% "
+                       "misses), 2 radix passes, planner bits", "build_rows": nr, "probe_rows": ns, "matches": m_want}
     for kind in ("fnv", "custom_xor_hw"):
-        opts = vjoin.join_opts(kind, 64, 64, passes=2)
         b, p = vjoin.table(dr), vjoin.table(ds)
+        opts = vjoin.join_opts(kind, 64, 64, passes=2)
+        plan = None
+        if fixed_plan:
+            opts = vjoin.join_opts(kind, 64, 64, passes=fixed_plan[0], bits1=fixed_plan[1], bits2=fixed_plan[2],
+                                   tile=fixed_plan[3], table=fixed_plan[4])
+        elif tune:
+            plan, _ = ctx.tune(b, p, opts, reps=3)
+            opts = vjoin.join_opts(kind, 64, 64, passes=plan[0], bits1=plan[1], bits2=plan[2], tile=plan[3],
+                                   table=plan[4])
         ms, stages = [], {k: [] for k in ("ms_histogram", "ms_pass1", "ms_pass2", "ms_build_probe")}
         for i in range(steps + 1):
             rc, cnt = ctx.join_into(b, p, opts, out_r, out_s)
@@ -645,6 +657,9 @@ def multicol_leg(ctx, steps, data):
         t = float(np.median(ms))
         B = 5 * (nr + ns) * 20 + m_want * 8
         res[kind] = {"ms": t, "stage_ms": {k[3:]: float(np.median(v)) for k, v in stages.items()},
+                     "plan": {"passes": sm["passes"], "bits": [sm["bits1"], sm["bits2"]], "tile_rows": sm["tile_rows"],
+                              "probe_threads": sm["probe_threads"], "table_rows": sm["table_rows"],
+                              "autotuned": plan is not None},
                      "probe_tuples_per_s": ns / (t * 1e-3), "algorithmic_bytes": B,
                      "achieved_GBps": B / (t * 1e-3) / 1e9, "frac": B / (t * 1e-3) / HBM_PEAK, "parity": bool(ok)}
     return res
@@ -756,6 +771,7 @@ def main():
     ap.add_argument("--batched-reps", type=int, default=3)
     ap.add_argument("--no-serving", action="store_true", help="skip the served-vs-plain request stream")
     ap.add_argument("--no-tune", action="store_true", help="use the planner's radix bits instead of vj_tune")
+    ap.add_argument("--mc-plan", default="", help="fixed multicol plan 'passes,bits1,bits2,tile,table' (profiling runs)")
     ap.add_argument("--no-multicol", action="store_true", help="skip the multi-column skewed join leg")
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--serving-rates", type=float, nargs="+", default=[2000.0, 8000.0, 32000.0])
@@ -828,7 +844,8 @@ def main():
     mc = multicol_data() if ("multicol" in legs or "hash" in legs) else None
     if "multicol" in legs:
         print("bench: multi-column skewed join leg", file=sys.stderr, flush=True)
-        out["multicol_skew"] = multicol_leg(ctx, 5, mc)
+        fixed = [int(x) for x in args.mc_plan.split(",")] if args.mc_plan else None
+        out["multicol_skew"] = multicol_leg(ctx, 5, mc, tune=not args.no_tune, fixed_plan=fixed)
     if "hash" in legs:
         print("bench: hash kernel leg", file=sys.stderr, flush=True)
         shapes = []

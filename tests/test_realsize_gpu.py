@@ -73,13 +73,17 @@ def multicol_full():
             "dr": workloads.to_device(rcols), "ds": workloads.to_device(scols)}
 
 
-@pytest.mark.parametrize("kind", ["fnv", "custom_xor_hw"])
-def test_multicol_skew_full_size_matches_oracle(ctx, multicol_full, kind):
+@pytest.mark.parametrize("kind,plan", [("fnv", None), ("custom_xor_hw", None), ("fnv", (5, 6, 1, 2)),
+                                       ("custom_xor_hw", (5, 6, 2, 2))])
+def test_multicol_skew_full_size_matches_oracle(ctx, multicol_full, kind, plan):
+    """The planner's plan, and the autotuner's kind of plan (fewer bits, wide
+    tables, either pass tile)."""
     w = multicol_full
     m = w["m"]
     assert len(w["rcols"][0]) == 1 << 22 and len(w["scols"][0]) == 1 << 25
     out_r, out_s = _out(m)
-    opts = vjoin.join_opts(kind, 64, 64, passes=2)
+    opts = vjoin.join_opts(kind, 64, 64, passes=2) if plan is None else \
+        vjoin.join_opts(kind, 64, 64, passes=2, bits1=plan[0], bits2=plan[1], tile=plan[2], table=plan[3])
     rc, cnt = ctx.join_into(vjoin.table(w["dr"]), vjoin.table(w["ds"]), opts, out_r, out_s)
     assert rc == 0, ctx.last_error()
     assert cnt == m

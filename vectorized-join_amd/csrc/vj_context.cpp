@@ -88,9 +88,9 @@ void free_all(vj_ctx *c) {
     free_buf(c->meta);
 }
 
-// Grows `b` to at least `bytes`. Without a budget an arena that must grow
-// takes 1/8 more, so slowly growing shapes do not reallocate every time; under
-// a budget it takes exactly what admit() counted.
+// Grows `b` to at least `bytes`; with `slack` an arena that must grow takes
+// 1/8 more, so slowly growing shapes do not reallocate every time (a
+// reallocation frees the old buffer, which waits for the device).
 Err ensure(DevBuf &b, size_t bytes, bool slack) {
     if (b.bytes >= bytes) return Err::ok();
     free_buf(b);
@@ -534,8 +534,21 @@ void place_tuples(Plan &pl, char *tb) {
     }
 }
 
-Err allocate(vj_ctx *c, Plan &pl) {
-    const bool slack = c->budget == 0;
+// Bytes the arenas hold once those that must grow have grown with 1/8 slack.
+uint64_t grown_with_slack(const vj_ctx *c, const ArenaSizes &a) {
+    auto g = [](const DevBuf &b, uint64_t need) { return b.bytes >= need ? (uint64_t)b.bytes : align_up(need + need / 8); };
+    return g(c->tuples, a.tuples) + g(c->counts, a.counts) + g(c->base, a.base) + g(c->ptask, a.ptask) +
+           g(c->meta, a.meta);
+}
+
+// Without a budget the arenas that must grow take slack. Under a budget
+// (after admit()) they grow to exactly the plan, or, with `budget_slack`, with
+// slack where the budget leaves room for it beside `extra` bytes (a serving
+// lane, whose batches change size all the time; a direct context keeps the
+// room for a regrown output).
+Err allocate(vj_ctx *c, Plan &pl, bool budget_slack = false, uint64_t extra = 0) {
+    const bool slack =
+        c->budget == 0 || (budget_slack && grown_with_slack(c, arena_sizes(pl)) + extra <= c->budget);
     VJ_TRY(ensure(c->tuples, std::max<size_t>(pl.tuples_bytes, kAlign), slack));
     VJ_TRY(ensure(c->counts, std::max<size_t>(pl.counts_words * 4, kAlign), slack));
     VJ_TRY(ensure(c->base, std::max<size_t>(pl.base_words * 4, kAlign), slack));
@@ -917,8 +930,8 @@ namespace vj {
 Err pipeline_bytes(const std::vector<JoinSpec> &specs, const vj_join_opts *o, uint64_t *bytes) {
     Plan pl;
     VJ_TRY(build_join_plan(pl, specs, o));
-    // a lane launches under a budget, so its arenas are allocated at exactly
-    // these sizes (no growth slack)
+    // a lane launches under a budget: its arenas hold at least these sizes,
+    // and growth slack only where the budget leaves room for it
     *bytes = plan_scratch_bytes(pl) + pl.keep_bytes;
     return Err::ok();
 }
@@ -930,7 +943,7 @@ Err pipeline_launch(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts 
     VJ_TRY(build_join_plan(pl, specs, o, c));
     c->budget = budget;
     VJ_TRY(admit(c, pl, pl.keep_bytes));
-    VJ_TRY(allocate(c, pl));
+    VJ_TRY(allocate(c, pl, true, pl.keep_bytes));
     VJ_TRY(upload(ctx_arenas(c), pl, s));
     PipeArgs a = make_args(ctx_arenas(c), pl);
     VJ_TRY(launch_stages(c, pl, a, s, true));

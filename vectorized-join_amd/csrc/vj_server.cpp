@@ -53,6 +53,7 @@ struct Req {
     uint64_t cap = 0;
     bool keep = false;
     uint64_t need = 0;          // device bytes of this request alone
+    uint64_t rows = 0;          // build + probe rows (the size of its device work)
     vj_built *kept = nullptr;   // table being filled by the running launch
     bool done = false;
     vj_response resp{};
@@ -69,9 +70,16 @@ struct Lane {
     // reports complete) and must not be polled
     bool launching = false;
     std::vector<Req *> reqs;
+    uint64_t rows = 0;  // build + probe rows of the running batch
     size_t count_off = 0;
     uint64_t kept_bytes = 0;
 };
+
+// Batches of more rows than this do not share the GPU with each other: two
+// such joins side by side each take longer than alone (they contend for HBM
+// and CUs), while overlap only pays for hiding launch and host time, which a
+// join of this size (tens of microseconds of device work) already covers.
+constexpr uint64_t kOverlapRows = 1ull << 22;
 
 JoinSpec spec_of(const Req *r) {
     JoinSpec sp{r->has_build ? &r->build : nullptr, &r->probe, r->out_b, r->out_p, r->cap};
@@ -184,8 +192,10 @@ LaunchResult launch_batch(vj_server *sv, Lane &L, Lane &O, std::unique_lock<std:
     std::vector<Req *> batch;
     uint64_t need = 0;
     bool budget_cut = false;
+    const bool big_running = O.busy && O.rows > kOverlapRows;
     while (!sv->pending.empty() && (int)batch.size() < sv->max_batch) {
         Req *r = sv->pending.front();
+        if (big_running && r->rows > kOverlapRows) break;  // waits for the large batch (see try_launch)
         if (avail && need + r->need > avail) {
             budget_cut = true;
             break;
@@ -219,6 +229,8 @@ LaunchResult launch_batch(vj_server *sv, Lane &L, Lane &O, std::unique_lock<std:
     L.launching = true;
     sv->launch_in_progress = true;
     L.reqs = batch;
+    L.rows = 0;
+    for (Req *r : batch) L.rows += r->rows;
     sv->st.batches++;
     if (O.busy) sv->st.overlapped_batches++;
     if (batch.size() > sv->st.max_batch) sv->st.max_batch = batch.size();
@@ -254,6 +266,9 @@ bool try_launch(vj_server *sv, std::unique_lock<std::mutex> &lk) {
     Lane *F = !sv->lanes[0].busy ? &sv->lanes[0] : (!sv->lanes[1].busy ? &sv->lanes[1] : nullptr);
     if (!F) return false;
     Lane &O = F == &sv->lanes[0] ? sv->lanes[1] : sv->lanes[0];
+    // a large request waits for a large running batch to finish (it then
+    // goes out at once, batched with whatever queued behind it)
+    if (O.busy && O.rows > kOverlapRows && sv->pending.front()->rows > kOverlapRows) return false;
     return launch_batch(sv, *F, O, lk) != WAITING;
 }
 
@@ -344,6 +359,7 @@ vj_status vj_server_submit(vj_server *sv, const vj_request *rq, uint64_t *ticket
     r->out_p = rq->out_probe;
     r->cap = rq->capacity;
     r->keep = rq->keep_build != 0;
+    r->rows = (r->has_build ? r->build.n_rows : 0) + r->probe.n_rows;
     std::unique_lock<std::mutex> lk(sv->m);
     if (!r->has_build) {
         auto it = sv->tables.find(rq->table);

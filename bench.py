@@ -773,7 +773,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="use the planner's radix bits instead of vj_tune")
     ap.add_argument("--mc-plan", default="", help="fixed multicol plan 'passes,bits1,bits2,tile,table' (profiling runs)")
     ap.add_argument("--no-multicol", action="store_true", help="skip the multi-column skewed join leg")
-    ap.add_argument("--serving-requests", type=int, default=512)
+    ap.add_argument("--serving-requests", type=int, default=2048,
+                    help="requests per arrival rate (p99 is the 20th-worst of 2048)")
     ap.add_argument("--serving-rates", type=float, nargs="+", default=[2000.0, 8000.0, 32000.0])
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device three times (counter calibration)")

@@ -268,7 +268,7 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>()))
             }
         }
         uint32_t total;
-        const uint32_t my_loc = block_excl_scan<uint32_t>(cnt, sh_scan, &total);
+        const uint32_t my_loc = block_excl_scan<uint32_t, kPassWaves>(cnt, sh_scan, &total);
         if (tid < bins) {
             loc[tid] = my_loc;
             gdo[tid] = gdst[tid] - my_loc;  // output row of staged slot k = gdo[digit] + k

@@ -80,26 +80,24 @@ template <typename T> __device__ __forceinline__ T wave_incl_scan(T v) {
     return v;
 }
 
-// Exclusive scan over the block; `sh` holds >= blockDim/64 + 1 values.
-template <typename T> __device__ T block_excl_scan(T v, T *sh, T *total) {
-    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    T inc = wave_incl_scan(v);
+// Exclusive scan over the block; `sh` holds >= blockDim/64 values. Every wave
+// adds up the wave totals below it itself (no serial step); NW: waves per
+// block when known at compile time.
+template <typename T, int NW = 0> __device__ T block_excl_scan(T v, T *sh, T *total) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t nw = NW ? (uint32_t)NW : (blockDim.x + 63) >> 6;
+    const T inc = wave_incl_scan(v);
     if (lane == 63) sh[w] = inc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        T acc = 0;
-        for (uint32_t i = 0; i < nw; i++) {
-            T t = sh[i];
-            sh[i] = acc;
-            acc += t;
-        }
-        sh[nw] = acc;
+    T carry = 0, tot = 0;
+    for (uint32_t k = 0; k < nw; k++) {
+        const T x = sh[k];
+        carry += k < w ? x : T(0);
+        tot += x;
     }
-    __syncthreads();
-    T r = sh[w] + inc - v;
-    *total = sh[nw];
-    __syncthreads();
-    return r;
+    __syncthreads();  // the caller may reuse sh at once
+    *total = tot;
+    return carry + inc - v;
 }
 
 }  // namespace

@@ -446,8 +446,9 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
     constexpr int TH = ProbeShape<PV>::TH, ITERS = ProbeShape<PV>::ITERS;
     constexpr int U = kProbeUnroll;
     // Next-iteration loads double the key registers; they are used where the
-    // kernel stays within its register budget (single 8-byte key, FNV / XOR).
-    constexpr bool PF = KW * NC <= 8 && HP != HP_GENERIC;
+    // kernel stays within its register budget (single 8-byte key, or the wide
+    // block's 128 VGPRs; FNV / XOR).
+    constexpr bool PF = (KW * NC <= 8 || PV == 1) && HP != HP_GENERIC;
     const uint32_t bmask = wa.bmask, shift = wa.shift, s_end = wa.s_end;
     uint32_t total = 0;  // this lane's matches
     Key<KW, NC> nk[U];
@@ -769,8 +770,8 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
             em.pos = t.out_base;
             for (uint32_t k = 0; k < w; k++) em.pos += t.wave_count[k];
             // row-id groups of two iterations where registers allow (single 8-byte key, FNV / XOR), of
-            // four where the build rows' ids come from LDS (keys wider than 8 bytes)
-            replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 2 : (IDS ? 4 : 1), PV, IDS>(wa, nxt, rid, res, em);
+            // four (eight in the wide block) where the build rows' ids come from LDS (keys wider than 8 bytes)
+            replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 2 : (IDS ? (PV ? 8 : 4) : 1), PV, IDS>(wa, nxt, rid, res, em);
         }
         __syncthreads();
     }

@@ -634,11 +634,12 @@ def multicol_leg(ctx, steps, data, tune=True, fixed_plan=None):
         plan = None
         if fixed_plan:
             opts = vjoin.join_opts(kind, 64, 64, passes=fixed_plan[0], bits1=fixed_plan[1], bits2=fixed_plan[2],
-                                   tile=fixed_plan[3], table=fixed_plan[4])
+                                   tile=fixed_plan[3], table=fixed_plan[4],
+                                   layout=fixed_plan[5] if len(fixed_plan) > 5 else 0)
         elif tune:
             plan, _ = ctx.tune(b, p, opts, reps=3)
             opts = vjoin.join_opts(kind, 64, 64, passes=plan[0], bits1=plan[1], bits2=plan[2], tile=plan[3],
-                                   table=plan[4])
+                                   table=plan[4], layout=plan[5])
         ms, stages = [], {k: [] for k in ("ms_histogram", "ms_pass1", "ms_pass2", "ms_build_probe")}
         for i in range(steps + 1):
             rc, cnt = ctx.join_into(b, p, opts, out_r, out_s)
@@ -771,7 +772,10 @@ def main():
     ap.add_argument("--batched-reps", type=int, default=3)
     ap.add_argument("--no-serving", action="store_true", help="skip the served-vs-plain request stream")
     ap.add_argument("--no-tune", action="store_true", help="use the planner's radix bits instead of vj_tune")
-    ap.add_argument("--mc-plan", default="", help="fixed multicol plan 'passes,bits1,bits2,tile,table' (profiling runs)")
+    ap.add_argument("--mc-plan", default="",
+                    help="fixed multicol plan 'passes,bits1,bits2,tile,table[,layout]' (profiling runs)")
+    ap.add_argument("--layout", type=int, default=0,
+                    help="partition layout of the FK join with --no-tune (0 planner, 1 exact, 2 regions)")
     ap.add_argument("--no-multicol", action="store_true", help="skip the multi-column skewed join leg")
     ap.add_argument("--serving-requests", type=int, default=2048,
                     help="requests per arrival rate (p99 is the 20th-worst of 2048)")
@@ -903,13 +907,13 @@ def fk_leg(ctx, args, workload_tag, keep_host):
     del rcols, scols
     build, probe = vjoin.table(dr), vjoin.table(ds)
     if args.no_tune:  # planner bits (profiling runs: no extra joins in the trace)
-        opts = vjoin.join_opts("fnv", 64, 64, passes=args.passes)
+        opts = vjoin.join_opts("fnv", 64, 64, passes=args.passes, layout=args.layout)
     else:  # radix bits autotuned for the configured pass count (vj_tune, on these tables)
-        (tp, tb1, tb2, tt, ttab), tune_ms = ctx.tune(build, probe, vjoin.join_opts("fnv", 64, 64, passes=args.passes),
-                                                     reps=3)
-        opts = vjoin.join_opts("fnv", 64, 64, passes=tp, bits1=tb1, bits2=tb2, tile=tt, table=ttab)
-        print("bench: autotuned plan passes=%d bits=%d+%d tile=%d table=%d (%.3f ms)" % (tp, tb1, tb2, tt, ttab, tune_ms),
-              file=sys.stderr, flush=True)
+        (tp, tb1, tb2, tt, ttab, tlay), tune_ms = ctx.tune(build, probe,
+                                                           vjoin.join_opts("fnv", 64, 64, passes=args.passes), reps=3)
+        opts = vjoin.join_opts("fnv", 64, 64, passes=tp, bits1=tb1, bits2=tb2, tile=tt, table=ttab, layout=tlay)
+        print("bench: autotuned plan passes=%d bits=%d+%d tile=%d table=%d layout=%d (%.3f ms)"
+              % (tp, tb1, tb2, tt, ttab, tlay, tune_ms), file=sys.stderr, flush=True)
     t_alloc = time.perf_counter()
     out_r = torch.empty(ns, dtype=torch.int32, device="cuda")
     out_s = torch.empty(ns, dtype=torch.int32, device="cuda")

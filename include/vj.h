@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define VJ_ABI_VERSION 3
+#define VJ_ABI_VERSION 4
 #define VJ_MAX_KEY_COLS 4
 #define VJ_MAX_ROWS 4294901760ull /* 2^32 - 2^16 */
 
@@ -147,6 +147,8 @@ typedef struct vj_join_opts {
     uint64_t out_capacity_hint; /* expected pair count for vj_join (0 = probe rows) */
     int32_t tile;       /* radix-pass tile: 0 = planner / autotuned, VJ_TILE_STANDARD or VJ_TILE_SMALL */
     int32_t table;      /* build+probe block: 0 = planner / autotuned, VJ_TABLE_STANDARD or VJ_TABLE_WIDE */
+    int32_t layout;     /* partition layout: 0 = planner / autotuned, VJ_LAYOUT_EXACT or VJ_LAYOUT_REGIONS */
+    int32_t reserved;   /* 0 */
 } vj_join_opts;
 
 /* Kernel shapes (vj_join_opts.tile / .table). All joins of one pipeline (a
@@ -159,6 +161,23 @@ typedef struct vj_join_opts {
                                 (keys <= 8 bytes) or 1024 rows */
 #define VJ_TABLE_WIDE 2      /* 1024 threads (16 waves), one block per CU, LDS table of 8192 rows
                                 (keys <= 8 bytes), 4096 (<= 24 bytes) or 2048 */
+
+/* Partition layouts (vj_join_opts.layout). The planner takes the exact one;
+ * vj_tune times both for two-pass plans.
+ * EXACT: a histogram read of the keys gives every row its exact destination
+ * (partitions back to back; no device atomics).
+ * REGIONS: no histogram read. Each partition gets a region of fixed capacity
+ * (its mean size plus a margin of 32 standard deviations of a uniform spread,
+ * about 1.2-2x the mean) and the radix passes reserve their runs in it with
+ * device atomics. Faster where keys spread evenly over the partitions (the
+ * histogram is about a tenth of a large join); when a region overflows (heavy
+ * key skew) the join is run again with the exact layout, the summary reports
+ * it (layout_fallback), and the context uses the exact layout for that size
+ * class from then on. Two-pass plans with at least 3 bits in the first pass
+ * only, and not for batches that hold resident or kept build sides, served
+ * batches or vj_join_into_noalloc: these take the exact layout. */
+#define VJ_LAYOUT_EXACT 1
+#define VJ_LAYOUT_REGIONS 2
 
 void vj_join_opts_default(vj_join_opts *opts);
 
@@ -263,7 +282,7 @@ typedef struct vj_join_summary {
     double ms_histogram, ms_pass1, ms_pass2, ms_build_probe;
     uint64_t scratch_bytes;
     int32_t graph_replay;  /* 1: the join ran as a replay of a captured hipGraph (same launch shape as before) */
-    int32_t reserved;
+    int32_t layout_fallback; /* 1: a region of VJ_LAYOUT_REGIONS overflowed; the join ran again with the exact layout */
     /* Algorithmic HBM bytes of each stage -- the key bytes and row ids of every
      * tuple the stage reads and writes, plus 8 bytes per output pair -- and the
      * resulting GB/s over the stage's device time (0 while per-stage timing is
@@ -276,7 +295,7 @@ typedef struct vj_join_summary {
     int32_t tile_rows;         /* rows per radix-pass tile */
     int32_t probe_threads;     /* threads per build+probe block */
     int32_t table_rows;        /* rows per build chunk (LDS table capacity) */
-    int32_t reserved2;
+    int32_t layout;            /* VJ_LAYOUT_EXACT or VJ_LAYOUT_REGIONS: the layout the join's result came from */
 } vj_join_summary;
 
 vj_status vj_join_summary_get(vj_ctx *ctx, vj_join_summary *summary);
@@ -318,7 +337,8 @@ vj_status vj_join_batch(vj_ctx *ctx, int32_t n_joins, const vj_table *builds, co
  * a configuration value, join_provider.cuh:296-300, as are its threads and
  * rows per thread, ProbeConfig, join_provider.cu:177-192). Times every candidate plan near the
  * planner's choice for each table shape (each pass count; radix bits -1..+1;
- * both splits of two passes; standard and wide build+probe blocks), then the
+ * both splits of two passes, each in both partition layouts; standard and wide
+ * build+probe blocks), then the
  * small pass tile with the fastest, on the caller's tables, `reps` timed runs
  * each after a warm-up, and keeps the fastest in the context for the size
  * class of `build` (build rows rounded up to a power of two, key bytes, hash
@@ -326,14 +346,15 @@ vj_status vj_join_batch(vj_ctx *ctx, int32_t n_joins, const vj_table *builds, co
  * use the tuned plan for their size class (vj_join, vj_join_into,
  * vj_join_batch; a batch takes its shapes from its largest build's class).
  * With opts->passes >= 0 only plans of that pass count are tried; a nonzero
- * opts->tile or opts->table fixes that shape.
- * *best receives opts with the chosen passes/bits/tile/table, *best_ms its
+ * opts->tile, opts->table or opts->layout fixes that shape or layout.
+ * *best receives opts with the chosen passes/bits/tile/table/layout, *best_ms its
  * median time.
  * Allocates a temporary output; synchronous.
  */
 vj_status vj_tune(vj_ctx *ctx, const vj_table *build, const vj_table *probe, const vj_join_opts *opts, int32_t reps,
                   vj_join_opts *best, double *best_ms, void *stream);
-/* Forgets every tuned plan of the context. */
+/* Forgets every tuned plan of the context (and which size classes overflowed
+ * a region of VJ_LAYOUT_REGIONS). */
 vj_status vj_tune_clear(vj_ctx *ctx);
 
 /* ------------------------------------------------------------------------ */

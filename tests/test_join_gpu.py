@@ -406,13 +406,15 @@ def test_tune_keeps_fastest_plan_and_results():
     c = vjoin.Context(0)
     try:
         c.set_profiling(True)
-        (P, b1, b2, tile, table), ms = c.tune(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts(), reps=2)
+        (P, b1, b2, tile, table, layout), ms = c.tune(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts(), reps=2)
         assert ms > 0 and P in (0, 1, 2)
         assert (P == 0 and b1 == 0) or (P >= 1 and 1 <= b1 <= 8) and (P < 2 or 1 <= b2 <= 8)
         assert tile in (vjoin.TILE_STANDARD, vjoin.TILE_SMALL) and table in (vjoin.TABLE_STANDARD, vjoin.TABLE_WIDE)
+        assert layout == vjoin.LAYOUT_EXACT or (P == 2 and layout == vjoin.LAYOUT_REGIONS)
         r, s_ = c.join(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts())
         sm = c.summary()
         assert (sm["passes"], sm["bits1"], sm["bits2"] if P == 2 else 0) == (P, b1, b2 if P == 2 else 0)
+        assert sm["layout"] == layout and sm["layout_fallback"] == 0
         # the tuned kernel shapes are used too (4-byte keys: tiles of 3072 or 1024 rows, tables of 2048 or 8192)
         assert sm["tile_rows"] == (3072 if tile == vjoin.TILE_STANDARD else 1024)
         assert (sm["probe_threads"], sm["table_rows"]) == ((512, 2048) if table == vjoin.TABLE_STANDARD else (1024, 8192))

@@ -11,7 +11,9 @@
   * resident 2-pass build sides re-probed by 2^18..2^22-row probes, directly and
     through the serving engine mixed with fresh joins, against the numpy join;
   * a 2^26 x 2^27 FK join of int64 keys (past the standard plan's 14 bits) with
-    the wide build+probe block and with the standard one, against the oracle.
+    the wide build+probe block and with the standard one, against the oracle;
+  * the 2^24 x 2^27 FK join in the region layout against the oracle, and the
+    multi-column Zipf join in it (whose regions overflow: exact-layout rerun).
 
 Pairs are compared as sorted (build_rowid << 32 | probe_rowid) keys; the sort
 runs on the device (torch.sort) only to compare, the pairs come from libvj.so
@@ -74,25 +76,56 @@ def multicol_full():
 
 
 @pytest.mark.parametrize("kind,plan", [("fnv", None), ("custom_xor_hw", None), ("fnv", (5, 6, 1, 2)),
-                                       ("custom_xor_hw", (5, 6, 2, 2))])
+                                       ("custom_xor_hw", (5, 6, 2, 2)), ("fnv", (5, 6, 1, 2, 2))])
 def test_multicol_skew_full_size_matches_oracle(ctx, multicol_full, kind, plan):
     """The planner's plan, and the autotuner's kind of plan (fewer bits, wide
-    tables, either pass tile)."""
+    tables, either pass tile). In the region layout the Zipf keys overflow
+    their regions and the join runs again with the exact layout."""
     w = multicol_full
     m = w["m"]
     assert len(w["rcols"][0]) == 1 << 22 and len(w["scols"][0]) == 1 << 25
     out_r, out_s = _out(m)
+    layout = plan[4] if plan is not None and len(plan) > 4 else 0
     opts = vjoin.join_opts(kind, 64, 64, passes=2) if plan is None else \
-        vjoin.join_opts(kind, 64, 64, passes=2, bits1=plan[0], bits2=plan[1], tile=plan[2], table=plan[3])
+        vjoin.join_opts(kind, 64, 64, passes=2, bits1=plan[0], bits2=plan[1], tile=plan[2], table=plan[3],
+                        layout=layout)
     rc, cnt = ctx.join_into(vjoin.table(w["dr"]), vjoin.table(w["ds"]), opts, out_r, out_s)
     assert rc == 0, ctx.last_error()
     assert cnt == m
     sm = ctx.summary()
     assert sm["passes"] == 2
+    if layout == vjoin.LAYOUT_REGIONS:
+        assert (sm["layout"], sm["layout_fallback"]) == (vjoin.LAYOUT_EXACT, 1)
     wr, ws = oracle.join(w["r_rows"], w["s_rows"], kind, 64, 64, passes=2, bits1=sm["bits1"], bits2=sm["bits2"],
                          cap=m)
     assert len(wr) == m
     _assert_same(_dev_keys(out_r, out_s, cnt), _host_keys(wr, ws), kind)
+
+
+# --- the large FK join in the region layout: 2^24 x 2^27 -----------------------------
+
+
+def test_large_fk_region_layout_matches_oracle(ctx):
+    """The large FK join (build 2^24 unique int64 keys, probe 2^27 drawn from
+    them, FNV-1a-64, 2 passes of 7 bits) with the region layout: no fall back,
+    every probe row matched once, the oracle's pairs."""
+    import torch
+    rcols, scols = workloads.large_fk(log_r=24, log_s=27, seed=24)
+    ns = len(scols[0])
+    dr, ds = workloads.to_device(rcols), workloads.to_device(scols)
+    out_r, out_s = _out(ns)
+    opts = vjoin.join_opts("fnv", 64, 64, passes=2, bits1=7, bits2=7, layout=vjoin.LAYOUT_REGIONS)
+    rc, cnt = ctx.join_into(vjoin.table(dr), vjoin.table(ds), opts, out_r, out_s)
+    assert rc == 0, ctx.last_error()
+    sm = ctx.summary()
+    assert cnt == ns and (sm["layout"], sm["layout_fallback"]) == (vjoin.LAYOUT_REGIONS, 0)
+    got = _dev_keys(out_r, out_s, cnt)
+    wr, ws = oracle.join(rcols[0].view(np.uint64).reshape(-1, 1), scols[0].view(np.uint64).reshape(-1, 1), "fnv",
+                         64, 64, passes=2, bits1=7, bits2=7, cap=ns)
+    assert len(wr) == ns
+    _assert_same(got, _host_keys(wr, ws), "regions")
+    del out_r, out_s, dr, ds, got
+    torch.cuda.empty_cache()
 
 
 # --- a build side past the 14-bit two-pass plan: 2^26 x 2^27 FK join ------------------

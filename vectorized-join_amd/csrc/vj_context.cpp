@@ -11,7 +11,9 @@
 #include <algorithm>
 #include <array>
 #include <string>
+#include <cmath>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "vj_host.hpp"
@@ -56,8 +58,10 @@ struct vj_ctx {
     std::vector<uint64_t> last_sig;  // signature of the last eager pipeline (automatic capture)
     bool graph_failed = false;       // capture failed once: stay eager
     uint64_t graph_replays = 0, graph_captures = 0;
-    // Autotuned plans (vj_tune): size class -> tuned_word(passes, bits1, bits2, tile, table)
-    std::unordered_map<uint64_t, uint32_t> tuned;
+    // Autotuned plans (vj_tune): size class -> tuned_word(passes, bits1, bits2, tile, table, layout)
+    std::unordered_map<uint64_t, uint64_t> tuned;
+    // size classes whose tuned region layout overflowed: the exact layout from then on
+    std::unordered_set<uint64_t> region_overflowed;
     // vj_join_into_noalloc: this device's fixed scratch (looked up at creation)
     // and a pinned staging block for its descriptors and counts (allocated at
     // creation, so the entry point itself allocates nothing)
@@ -124,6 +128,8 @@ struct Plan {
     std::vector<ColScanDesc> colscan;
     std::vector<uint32_t> hist_block0, p1_tile0, p2_chunk0;
     std::vector<size_t> fin_off, mid_off;  // byte offsets in the tuples arena (SIZE_MAX: none)
+    std::vector<uint64_t> fin_rows, mid_rows;  // rows of each buffer (the region layout leaves gaps)
+    bool regions = false;                  // region layout (every job has two passes)
     size_t tuples_bytes = 0;
     uint64_t counts_words = 0, base_words = 0, ptask_words = 0;
     uint64_t taskmap_cap = 0;  // entries of the task map (after the task prefixes in the ptask arena)
@@ -235,6 +241,7 @@ uint32_t add_resident_job(Plan &pl, const vj_built *b) {
     jd.fin_ids = b->ids;
     jd.res_base = b->base;
     jd.base_off = (uint32_t)pl.base_words;
+    jd.end_off = jd.base_off + 1;
     pl.base_words += (1u << (b->b1 + b->b2)) + 1;
     pl.any_part = true;
     pl.hist_block0.push_back(pl.total_hist_blocks);
@@ -242,8 +249,50 @@ uint32_t add_resident_job(Plan &pl, const vj_built *b) {
     pl.p2_chunk0.push_back(pl.total_p2_chunks);
     pl.fin_off.push_back(SIZE_MAX);
     pl.mid_off.push_back(SIZE_MAX);
+    pl.fin_rows.push_back(0);
+    pl.mid_rows.push_back(0);
     pl.jobs.push_back(jd);
     return (uint32_t)pl.jobs.size() - 1;
+}
+
+// Region layout: pass-1 tiles per segment (each of a segment's 2^b1 region
+// counters takes one device atomic per tile; a counter serialises its
+// atomics, so segments keep each one to about a thousand).
+constexpr uint32_t kRegionSegTiles = 1024;
+
+// Rows of a region whose expected fill is `mean` rows: the mean plus 32
+// standard deviations of a uniform (Poisson) spread plus 256, in whole
+// 64-row groups. Keys repeated a few times widen the spread by about the
+// square root of the repeat count (FK probe sides: 8 repeats, ~11 standard
+// deviations left); heavier skew overflows and falls back to the exact layout.
+uint64_t region_rows(uint64_t mean) {
+    const uint64_t r = mean + (uint64_t)(32.0 * std::sqrt((double)mean)) + 256;
+    return (r + 63) & ~63ull;
+}
+
+// Region sizes of a two-pass job of n rows (false: its row indexes would not
+// fit 32 bits).
+struct RegionShape {
+    uint32_t nseg, seg_tiles, region1, region2, region_tiles;
+    uint64_t mid_rows, fin_rows;
+};
+bool region_shape(uint64_t n, int b1, int b2, uint32_t T, RegionShape *r) {
+    const uint64_t tiles = (n + T - 1) / T;
+    const uint64_t nseg = std::max<uint64_t>(1, std::min<uint64_t>((tiles + kRegionSegTiles - 1) / kRegionSegTiles,
+                                                                   kMaxSegments));
+    const uint64_t seg_tiles = std::max<uint64_t>(1, (tiles + nseg - 1) / nseg);
+    const uint64_t seg_rows = std::min<uint64_t>(n, seg_tiles * T);
+    const uint64_t r1 = region_rows((seg_rows + (1ull << b1) - 1) >> b1);
+    const uint64_t r2 = region_rows((n + (1ull << (b1 + b2)) - 1) >> (b1 + b2));
+    r->nseg = (uint32_t)((tiles + seg_tiles - 1) / seg_tiles);
+    if (r->nseg == 0) r->nseg = 1;
+    r->seg_tiles = (uint32_t)seg_tiles;
+    r->mid_rows = ((uint64_t)r->nseg << b1) * r1 + T;
+    r->fin_rows = (1ull << (b1 + b2)) * r2 + T;
+    r->region1 = (uint32_t)r1;
+    r->region2 = (uint32_t)r2;
+    r->region_tiles = (uint32_t)((r1 + T - 1) / T);
+    return r->mid_rows < (1ull << 32) && r->fin_rows < (1ull << 32);
 }
 
 void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0, bool fin_external = false) {
@@ -257,10 +306,37 @@ void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0
     const uint32_t T = pass_tile(pl.kb, pl.shape.tile);
     const uint32_t n1 = 1u << jd.b1, nb = 1u << (jd.b1 + jd.b2);
     jd.base_off = (uint32_t)pl.base_words;
+    jd.end_off = jd.base_off + 1;
     pl.base_words += (jd.b1 ? nb : 1) + 1;
+    if (pl.regions && jd.b2) {  // region layout: the partition ends after the starts
+        jd.end_off = (uint32_t)pl.base_words;
+        pl.base_words += nb;
+    }
     const uint32_t job_index = (uint32_t)pl.jobs.size();
     uint32_t chunks = 0;
-    if (jd.b1 && jd.n) {
+    uint64_t mid_rows = jd.n, fin_rows = jd.n;
+    if (pl.regions && jd.b2) {
+        // region layout: no histogram rows, no column scans; counters m1
+        // [2^b1][nseg] and m2 [2^(b1+b2)] (zeroed by k_region_reset)
+        RegionShape rs;
+        (void)region_shape(jd.n, (int)jd.b1, (int)jd.b2, T, &rs);  // checked by the planner
+        pl.any_part = true;
+        pl.any_p2 = true;
+        jd.p1_tiles = (uint32_t)(((uint64_t)jd.n + T - 1) / T);
+        jd.nseg = rs.nseg;
+        jd.seg_tiles = rs.seg_tiles;
+        jd.hsplit = 1;
+        jd.region1 = rs.region1;
+        jd.region2 = rs.region2;
+        jd.region_tiles = jd.n ? rs.region_tiles : 0;  // an empty job has no pass-2 chunks
+        jd.m1_off = pl.counts_words;
+        pl.counts_words += (uint64_t)jd.nseg * n1;
+        jd.m2_off = pl.counts_words;
+        pl.counts_words += nb;
+        chunks = jd.nseg * n1 * jd.region_tiles;
+        mid_rows = rs.mid_rows;
+        fin_rows = rs.fin_rows;
+    } else if (jd.b1 && jd.n) {
         pl.any_part = true;
         jd.p1_tiles = (uint32_t)(((uint64_t)jd.n + T - 1) / T);
         // segments: pass-2 chunks (a segment's rows of one pass-1 partition)
@@ -306,19 +382,23 @@ void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0
     pl.total_p2_chunks += chunks;
     // tuple buffers: fin (and mid for two passes), SoA key columns + row ids
     size_t fin = SIZE_MAX, mid = SIZE_MAX;
+    auto bytes = [&](uint64_t rows) {
+        return align_up((size_t)t->key_width * rows) * t->n_cols + align_up((size_t)4 * rows);
+    };
     if (jd.b1) {
-        const size_t one = align_up((size_t)t->key_width * jd.n) * t->n_cols + align_up((size_t)4 * jd.n);
         if (!fin_external) {
             fin = pl.tuples_bytes;
-            pl.tuples_bytes += one;
+            pl.tuples_bytes += bytes(fin_rows);
         }
         if (jd.b2) {
             mid = pl.tuples_bytes;
-            pl.tuples_bytes += one;
+            pl.tuples_bytes += bytes(mid_rows);
         }
     }
     pl.fin_off.push_back(fin);
     pl.mid_off.push_back(mid);
+    pl.fin_rows.push_back(fin_rows);
+    pl.mid_rows.push_back(mid_rows);
     pl.jobs.push_back(jd);
 }
 
@@ -358,15 +438,26 @@ uint64_t tune_key(uint64_t nr, int kb, int hp) {
     return (uint64_t)ceil_log2(std::max<uint64_t>(nr, 1)) | (uint64_t)kb << 8 | (uint64_t)hp << 16;
 }
 
-// Tuned plan word: passes | bits1 << 8 | bits2 << 16 | tile << 24 | table << 28.
-uint32_t tuned_word(int P, int b1, int b2, int tile, int table) {
-    return (uint32_t)P | (uint32_t)b1 << 8 | (uint32_t)b2 << 16 | (uint32_t)tile << 24 | (uint32_t)table << 28;
+// Tuned plan word: passes | bits1 << 8 | bits2 << 16 | tile << 24 | table << 28 | layout << 32
+// (tile, table: 0-based shapes; layout: VJ_LAYOUT_*).
+uint64_t tuned_word(int P, int b1, int b2, int tile, int table, int layout) {
+    return (uint64_t)P | (uint64_t)b1 << 8 | (uint64_t)b2 << 16 | (uint64_t)tile << 24 | (uint64_t)table << 28 |
+           (uint64_t)layout << 32;
 }
 
 Err check_shapes(const vj_join_opts *o) {
     if (o->tile < 0 || o->tile > VJ_TILE_SMALL || o->table < 0 || o->table > VJ_TABLE_WIDE)
         return Err::make(VJ_ERR_INVALID_ARGUMENT, "tile and table must be 0 (planner), 1 (standard) or 2");
+    if (o->layout < 0 || o->layout > VJ_LAYOUT_REGIONS)
+        return Err::make(VJ_ERR_INVALID_ARGUMENT, "layout must be 0 (planner), 1 (exact) or 2 (regions)");
     return Err::ok();
+}
+
+// Size class of a pipeline: its largest build side.
+uint64_t largest_build(const std::vector<JoinSpec> &specs) {
+    uint64_t nr = 0;
+    for (const JoinSpec &js : specs) nr = std::max<uint64_t>(nr, js.res ? js.res->n : js.r->n_rows);
+    return nr;
 }
 
 // Kernel shapes of a pipeline (one for all of its joins): the caller's, else
@@ -376,8 +467,7 @@ Err check_shapes(const vj_join_opts *o) {
 // half a chunk per partition.
 Err choose_shapes(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_opts *o, const vj_ctx *c) {
     VJ_TRY(check_shapes(o));
-    uint64_t nr = 0;
-    for (const JoinSpec &js : specs) nr = std::max<uint64_t>(nr, js.res ? js.res->n : js.r->n_rows);
+    const uint64_t nr = largest_build(specs);
     int tile = o->tile - 1, table = o->table - 1;  // -1: choose
     if (c && o->passes == -1 && !o->bits1 && !o->bits2 && !c->tuned.empty()) {
         auto it = c->tuned.find(tune_key(nr, pl.kb, pl.shape.hp));
@@ -397,8 +487,34 @@ bool same_hash(const vj_hash_opts &a, const vj_hash_opts &b) {
     return a.kind == b.kind && a.hash_bits == b.hash_bits && a.chunk_bits == b.chunk_bits && a.word_mode == b.word_mode;
 }
 
-Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_opts *o,
-                    const vj_ctx *c = nullptr) {
+// The partition layout of a pipeline: the caller's, else the autotuned one of
+// its size class (unless that class has overflowed a region before), else
+// exact; regions only where every join has two passes and no resident or
+// kept build side, and every region's rows fit 32-bit indexes.
+bool choose_regions(const Plan &pl, const std::vector<JoinSpec> &specs, const std::vector<std::array<int, 3>> &bits,
+                    const vj_join_opts *o, const vj_ctx *c, bool allow) {
+    int layout = o->layout;
+    if (!layout && c && o->passes == -1 && !o->bits1 && !o->bits2 && !c->tuned.empty()) {
+        const uint64_t key = tune_key(largest_build(specs), pl.kb, pl.shape.hp);
+        auto it = c->tuned.find(key);
+        if (it != c->tuned.end() && !c->region_overflowed.count(key)) layout = (int)((it->second >> 32) & 0xF);
+    }
+    if (layout != VJ_LAYOUT_REGIONS || !allow) return false;
+    const uint32_t T = pass_tile(pl.kb, pl.shape.tile);
+    for (size_t i = 0; i < specs.size(); i++) {
+        const JoinSpec &js = specs[i];
+        // pass 2 deals a job's digits to the XCDs: 2^b1 must be a multiple of kXcds
+        if (js.res || js.keep || bits[i][0] != 2 || (1u << bits[i][1]) % kXcds) return false;
+        RegionShape rs;
+        if (!region_shape(js.r->n_rows, bits[i][1], bits[i][2], T, &rs) ||
+            !region_shape(js.s->n_rows, bits[i][1], bits[i][2], T, &rs))
+            return false;
+    }
+    return true;
+}
+
+Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_opts *o, const vj_ctx *c = nullptr,
+                    bool allow_regions = true) {
     const vj_table *t0 = specs[0].s;
     VJ_TRY(check_table(t0));
     VJ_TRY(make_hash_cfg(&o->hash, t0, &pl.hash));
@@ -411,11 +527,11 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
     VJ_TRY(choose_shapes(pl, specs, o, c));
     for (const JoinSpec &js : specs) pl.hist_rows_total += (js.res ? 0 : js.r->n_rows) + js.s->n_rows;
     pl.hist_blocks_target = 2u * (uint32_t)(c ? c->cus : 256);  // sizing without a context: an MI355X
+    // radix plan of every join: (passes, bits1, bits2)
+    std::vector<std::array<int, 3>> bits(specs.size());
     for (size_t i = 0; i < specs.size(); i++) {
         const JoinSpec &js = specs[i];
-        int P, b1, b2;
-        JoinDesc jd;
-        memset(&jd, 0, sizeof(jd));
+        int &P = bits[i][0], &b1 = bits[i][1], &b2 = bits[i][2];
         if (js.res) {
             VJ_TRY(check_table(js.s));
             const vj_built *b = js.res;
@@ -427,7 +543,6 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
             P = b->b2 ? 2 : 1;
             b1 = (int)b->b1;
             b2 = (int)b->b2;
-            jd.job_r = add_resident_job(pl, b);
         } else {
             VJ_TRY(check_pair(js.r, js.s));
             VJ_TRY(plan_bits(js.r->n_rows, pl.kb, pl.shape.table, o, &P, &b1, &b2));
@@ -444,6 +559,17 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
                 b1 = 1;
                 b2 = 0;
             }
+        }
+    }
+    pl.regions = choose_regions(pl, specs, bits, o, c, allow_regions);
+    for (size_t i = 0; i < specs.size(); i++) {
+        const JoinSpec &js = specs[i];
+        const int P = bits[i][0], b1 = bits[i][1], b2 = bits[i][2];
+        JoinDesc jd;
+        memset(&jd, 0, sizeof(jd));
+        if (js.res) {
+            jd.job_r = add_resident_job(pl, js.res);
+        } else {
             jd.job_r = (uint32_t)pl.jobs.size();
             add_job(pl, js.r, P, b1, b2, 0, js.keep != nullptr);
             if (js.keep) {
@@ -520,13 +646,14 @@ Err admit(vj_ctx *c, const Plan &pl, uint64_t extra) {
 void place_tuples(Plan &pl, char *tb) {
     for (size_t j = 0; j < pl.jobs.size(); j++) {
         JobDesc &jd = pl.jobs[j];
-        const size_t col = align_up((size_t)pl.shape.kw * jd.n);
         if (pl.fin_off[j] != SIZE_MAX) {
+            const size_t col = align_up((size_t)pl.shape.kw * pl.fin_rows[j]);
             char *p = tb + pl.fin_off[j];
             for (int cc = 0; cc < 4; cc++) jd.fin_cols[cc] = cc < pl.shape.nc ? p + cc * col : nullptr;
             jd.fin_ids = reinterpret_cast<uint32_t *>(p + pl.shape.nc * col);
         }
         if (pl.mid_off[j] != SIZE_MAX) {
+            const size_t col = align_up((size_t)pl.shape.kw * pl.mid_rows[j]);
             char *p = tb + pl.mid_off[j];
             for (int cc = 0; cc < 4; cc++) jd.mid_cols[cc] = cc < pl.shape.nc ? p + cc * col : nullptr;
             jd.mid_ids = reinterpret_cast<uint32_t *>(p + pl.shape.nc * col);
@@ -611,6 +738,7 @@ PipeArgs make_args(const Arenas &ar, const Plan &pl) {
     a.total_p2_chunks = pl.total_p2_chunks;
     a.max_hist_bins = pl.max_hist_bins;
     a.pass_tile = pass_tile(pl.kb, pl.shape.tile);
+    a.regions = pl.regions ? 1u : 0u;
     a.hash = pl.hash;
     return a;
 }
@@ -644,14 +772,26 @@ Err upload(const Arenas &ar, const Plan &pl, hipStream_t s) {
 Err launch_stages(vj_ctx *c, const Plan &pl, const PipeArgs &a, hipStream_t s, bool with_probe, bool events = true) {
     const bool prof = c->profiling && events;
     if (events) VJ_HIP_TRY(hipEventRecord(c->ev[EV_START], s));
-    if (pl.any_part) VJ_LAUNCH(launch_hist(a, pl.shape, s));
-    if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_HIST], s));
-    VJ_LAUNCH(launch_scan(a, s));
-    if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_SCAN], s));
-    if (pl.any_part) VJ_LAUNCH(launch_pass(a, pl.shape, 1, s));
-    if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_P1], s));
-    if (pl.any_p2) VJ_LAUNCH(launch_pass(a, pl.shape, 2, s));
-    if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_P2], s));
+    if (pl.regions) {
+        // region layout: counters zeroed, both passes reserve runs, then the task prefix
+        VJ_LAUNCH(launch_region_reset(a, s));
+        if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_HIST], s));
+        if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_SCAN], s));
+        VJ_LAUNCH(launch_pass(a, pl.shape, 1, s));
+        if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_P1], s));
+        VJ_LAUNCH(launch_pass(a, pl.shape, 2, s));
+        if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_P2], s));
+        VJ_LAUNCH(launch_tasks(a, s));
+    } else {
+        if (pl.any_part) VJ_LAUNCH(launch_hist(a, pl.shape, s));
+        if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_HIST], s));
+        VJ_LAUNCH(launch_scan(a, s));
+        if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_SCAN], s));
+        if (pl.any_part) VJ_LAUNCH(launch_pass(a, pl.shape, 1, s));
+        if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_P1], s));
+        if (pl.any_p2) VJ_LAUNCH(launch_pass(a, pl.shape, 2, s));
+        if (prof) VJ_HIP_TRY(hipEventRecord(c->ev[EV_P2], s));
+    }
     if (with_probe && !pl.joins.empty())
         VJ_LAUNCH(launch_probe(a, pl.shape, c->cus * probe_blocks_per_cu(pl.shape), s));
     if (events) VJ_HIP_TRY(hipEventRecord(c->ev[EV_PROBE], s));
@@ -664,7 +804,7 @@ std::vector<uint64_t> graph_signature(const vj_ctx *c, const Plan &pl, const Pip
     std::vector<uint64_t> sig((sizeof(PipeArgs) + 7) / 8 + 10, 0);  // zeroed: PipeArgs padding is zero too
     memcpy(sig.data(), &a, sizeof(PipeArgs));
     uint64_t *x = sig.data() + (sizeof(PipeArgs) + 7) / 8;
-    x[0] = (uint64_t)pl.any_part | (uint64_t)pl.any_p2 << 1;
+    x[0] = (uint64_t)pl.any_part | (uint64_t)pl.any_p2 << 1 | (uint64_t)pl.regions << 2;
     x[1] = (uint64_t)pl.shape.hp | (uint64_t)pl.shape.kw << 8 | (uint64_t)pl.shape.nc << 16 |
            (uint64_t)pl.shape.tile << 24 | (uint64_t)pl.shape.table << 32;
     x[2] = (uint64_t)c->cus * (uint64_t)probe_blocks_per_cu(pl.shape);
@@ -738,7 +878,7 @@ void stage_bytes(const Plan &pl, uint64_t matches, vj_join_summary &s) {
         const uint64_t n = jd.n, ids_in = jd.in.rowids ? 4 : 0;
         const bool resident = jd.res_base != nullptr;
         if (jd.b1 && !resident) {
-            s.bytes_histogram += n * kb;
+            if (!pl.regions) s.bytes_histogram += n * kb;
             s.bytes_pass1 += n * (kb + ids_in) + n * (kb + 4);
         }
         if (jd.b2 && !resident) s.bytes_pass2 += 2 * n * (kb + 4);
@@ -776,6 +916,7 @@ void fill_summary(vj_ctx *c, const Plan &pl, const JoinSpec &js, uint64_t matche
     s.tile_rows = pass_tile(pl.kb, pl.shape.tile);
     s.probe_threads = probe_threads(pl.shape.table);
     s.table_rows = probe_cap(pl.kb, pl.shape.table);
+    s.layout = pl.regions ? VJ_LAYOUT_REGIONS : VJ_LAYOUT_EXACT;
     stage_bytes(pl, matches, s);
     s.gbps_histogram = gbps(s.bytes_histogram, s.ms_histogram);
     s.gbps_pass1 = gbps(s.bytes_pass1, s.ms_pass1);
@@ -847,6 +988,16 @@ Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hi
                                   hipMemcpyDeviceToHost, s));
     }
     VJ_HIP_TRY(hipStreamSynchronize(s));
+    if (pl.regions && reinterpret_cast<const Ctl *>(h + pl.off_ctl)->overflow) {
+        // a region overflowed (the keys do not spread evenly): the same joins
+        // with the exact layout; an autotuned size class keeps to it from now on
+        if (!o->layout) c->region_overflowed.insert(tune_key(largest_build(specs), pl.kb, pl.shape.hp));
+        vj_join_opts ox = *o;
+        ox.layout = VJ_LAYOUT_EXACT;
+        VJ_TRY(run_joins(c, specs, &ox, s, counts, grow, extra_bytes, retries_out, gmode));
+        c->summary.layout_fallback = 1;
+        return Err::ok();
+    }
     counts.assign(hc, hc + pl.joins.size());
     int retries = 0;
     if (grow) {
@@ -929,7 +1080,7 @@ namespace vj {
 
 Err pipeline_bytes(const std::vector<JoinSpec> &specs, const vj_join_opts *o, uint64_t *bytes) {
     Plan pl;
-    VJ_TRY(build_join_plan(pl, specs, o));
+    VJ_TRY(build_join_plan(pl, specs, o, nullptr, false));
     // a lane launches under a budget: its arenas hold at least these sizes,
     // and growth slack only where the budget leaves room for it
     *bytes = plan_scratch_bytes(pl) + pl.keep_bytes;
@@ -940,7 +1091,7 @@ Err pipeline_launch(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts 
                     size_t *count_off) {
     VJ_TRY(set_device(c));
     Plan pl;
-    VJ_TRY(build_join_plan(pl, specs, o, c));
+    VJ_TRY(build_join_plan(pl, specs, o, c, false));  // served batches complete asynchronously: exact layout
     c->budget = budget;
     VJ_TRY(admit(c, pl, pl.keep_bytes));
     VJ_TRY(allocate(c, pl, true, pl.keep_bytes));
@@ -1052,6 +1203,8 @@ void vj_join_opts_default(vj_join_opts *o) {
     o->out_capacity_hint = 0;
     o->tile = 0;
     o->table = 0;
+    o->layout = 0;
+    o->reserved = 0;
 }
 
 vj_status vj_join_into(vj_ctx *c, const vj_table *build, const vj_table *probe, const vj_join_opts *opts,
@@ -1097,7 +1250,7 @@ vj_status vj_join_into_noalloc(vj_ctx *c, const vj_table *build, const vj_table 
     hipStream_t s = (hipStream_t)stream;
     std::vector<JoinSpec> specs{{build, probe, out_build, out_probe, capacity}};
     Plan pl;
-    e = build_join_plan(pl, specs, opts, c);
+    e = build_join_plan(pl, specs, opts, c, false);  // exact layout: its scratch is bounded by the join's rows
     if (e.failed()) return fail(c, e);
     // the arenas, laid out back to back in the fixed scratch
     const ArenaSizes az = arena_sizes(pl);
@@ -1452,15 +1605,19 @@ vj_status vj_tune(vj_ctx *c, const vj_table *build, const vj_table *probe, const
     const uint64_t nr = build->n_rows;
     // Candidates: for each table shape, plans around the planner's choice for
     // that table's chunk (every pass count, bits -1..+1, both splits of two
-    // passes), with the standard tile; then the small tile with the fastest.
+    // passes; two-pass plans in both partition layouts), with the standard
+    // tile; then the small tile with the fastest.
     struct Cand {
-        int P, b1, b2, tile, table;
+        int P, b1, b2, tile, table, layout;
     };
     std::vector<Cand> cands;
     auto add = [&](Cand x) {
         for (auto &y : cands)
-            if (y.P == x.P && y.b1 == x.b1 && y.b2 == x.b2 && y.tile == x.tile && y.table == x.table) return;
+            if (y.P == x.P && y.b1 == x.b1 && y.b2 == x.b2 && y.tile == x.tile && y.table == x.table &&
+                y.layout == x.layout)
+                return;
         if (opts->passes >= 0 && x.P != opts->passes) return;
+        if (opts->layout && x.layout != opts->layout) return;
         cands.push_back(x);
     };
     const int tile0 = opts->tile ? opts->tile - 1 : 0;
@@ -1468,12 +1625,16 @@ vj_status vj_tune(vj_ctx *c, const vj_table *build, const vj_table *probe, const
         if (opts->table && table != opts->table - 1) continue;
         const int cap = probe_cap(kb, table);
         const int need = need_bits(nr, cap);
-        if (nr <= (uint64_t)cap) add({0, 0, 0, tile0, table});
-        for (int b = std::max(1, need - 1); b <= std::min(kMaxPassBits, need + 1); b++) add({1, b, 0, tile0, table});
+        if (nr <= (uint64_t)cap) add({0, 0, 0, tile0, table, VJ_LAYOUT_EXACT});
+        for (int b = std::max(1, need - 1); b <= std::min(kMaxPassBits, need + 1); b++)
+            add({1, b, 0, tile0, table, VJ_LAYOUT_EXACT});
         for (int tot = std::max(2, need - 1); tot <= std::min(kMaxHistBits, need + 1); tot++)
             for (int b1 : {(tot + 1) / 2, tot / 2}) {
                 const int b2 = tot - b1;
-                if (b1 >= 1 && b1 <= kMaxPassBits && b2 >= 1 && b2 <= kMaxPassBits) add({2, b1, b2, tile0, table});
+                if (b1 >= 1 && b1 <= kMaxPassBits && b2 >= 1 && b2 <= kMaxPassBits) {
+                    add({2, b1, b2, tile0, table, VJ_LAYOUT_EXACT});
+                    if ((1u << b1) % kXcds == 0) add({2, b1, b2, tile0, table, VJ_LAYOUT_REGIONS});
+                }
             }
     }
     if (cands.empty()) return fail(c, Err::make(VJ_ERR_UNSUPPORTED, "no candidate plan with that pass count"));
@@ -1481,6 +1642,7 @@ vj_status vj_tune(vj_ctx *c, const vj_table *build, const vj_table *probe, const
     std::vector<JoinSpec> specs{{build, probe, nullptr, nullptr, 0}};
     std::vector<uint64_t> counts;
     c->tuned.erase(tune_key(nr, kb, hp));
+    c->region_overflowed.erase(tune_key(nr, kb, hp));
     auto plan_opts = [&](const Cand &x) {
         vj_join_opts o2 = *opts;
         o2.passes = x.P;
@@ -1488,6 +1650,7 @@ vj_status vj_tune(vj_ctx *c, const vj_table *build, const vj_table *probe, const
         o2.bits2 = x.b2;
         o2.tile = x.tile + 1;
         o2.table = x.table + 1;
+        o2.layout = x.layout;
         return o2;
     };
     vj_join_opts oc = plan_opts(cands[0]);  // counting run: any valid plan
@@ -1539,7 +1702,8 @@ vj_status vj_tune(vj_ctx *c, const vj_table *build, const vj_table *probe, const
     (void)hipFree(ob);
     (void)hipFree(op);
     if (e.failed()) return fail(c, e);
-    c->tuned[tune_key(nr, kb, hp)] = tuned_word(best_c.P, best_c.b1, best_c.b2, best_c.tile, best_c.table);
+    c->tuned[tune_key(nr, kb, hp)] =
+        tuned_word(best_c.P, best_c.b1, best_c.b2, best_c.tile, best_c.table, best_c.layout);
     *best = plan_opts(best_c);
     if (best_ms) *best_ms = best_t;
     return VJ_OK;
@@ -1548,6 +1712,7 @@ vj_status vj_tune(vj_ctx *c, const vj_table *build, const vj_table *probe, const
 vj_status vj_tune_clear(vj_ctx *c) {
     if (!c) return VJ_ERR_INVALID_ARGUMENT;
     c->tuned.clear();
+    c->region_overflowed.clear();
     return VJ_OK;
 }
 

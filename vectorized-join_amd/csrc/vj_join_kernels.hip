@@ -7,13 +7,17 @@
 //   k_bases     per job: partition bases from the column totals; block 0
 //               also resets the task queue and the output counters
 //   k_tasks     per join: build/probe task prefix over partition pairs (one
-//               join: also the queue totals and the task map)
+//               join: also the queue totals and the task map); in the region
+//               layout also the overflow check
+//   k_region_reset  region layout: zeroes the region counters (replaces
+//               k_hist, k_colscan and k_bases)
 //   k_scan2     task prefix across joins                 (1 block, >1 join)
 //   k_taskmap   task -> (join, partition) for the first tasks  (>1 join)
 //   k_pass      one radix pass over a tile (pass 1) or a (digit, segment)
 //               chunk (pass 2): per-wave LDS digit ranks -> LDS staging
 //               ordered by digit -> contiguous runs per digit written at the
-//               precomputed destinations; no block waits on another
+//               precomputed destinations (region layout: at runs reserved with
+//               one device atomic per digit); no block waits on another
 //   k_probe     persistent; (join, partition, R-chunk, S-chunk) tasks from a
 //               device queue: open-addressing table of the R chunk in LDS,
 //               probe with the S chunk; matches counted first, one output
@@ -162,7 +166,10 @@ __global__ __launch_bounds__(1024) void k_colscan(PipeArgs a) {
 __global__ __launch_bounds__(1024) void k_bases(PipeArgs a) {
     __shared__ uint32_t sh32[20];
     if (blockIdx.x == 0) {
-        if (threadIdx.x == 0) a.ctl->task_ticket = 0;
+        if (threadIdx.x == 0) {
+            a.ctl->task_ticket = 0;
+            a.ctl->overflow = 0;
+        }
         for (uint32_t j = threadIdx.x; j < a.n_joins; j += blockDim.x) a.out_count[j] = 0;
     }
     if (blockIdx.x >= a.n_jobs) return;
@@ -225,28 +232,57 @@ __device__ __forceinline__ unsigned long long part_tasks(uint32_t nr, uint32_t n
     return nr && ns ? (unsigned long long)(((nr - 1) >> lr) + 1) * (((ns - 1) >> ls) + 1) : 0ull;
 }
 
+// Whether any region of a region-layout job received more rows than it holds
+// (block-wide; false for exact-layout jobs).
+__device__ bool region_overflow(const PipeArgs &a, const JobDesc &job) {
+    if (!job.region1) return false;
+    const gptr<const uint32_t> m1 = to_global(a.counts + job.m1_off), m2 = to_global(a.counts + job.m2_off);
+    const uint32_t n1 = job.nseg << job.b1, nb = 1u << (job.b1 + job.b2);
+    int over = 0;
+    for (uint32_t i = threadIdx.x; i < n1; i += blockDim.x) over |= m1[i] > job.region1;
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) over |= m2[i] > job.region2;
+    return __syncthreads_or(over) != 0;
+}
+
 __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
     __shared__ unsigned long long sh64[20];
     const uint32_t jn = blockIdx.x;
     const JoinDesc &jd = a.joins[jn];
-    const gptr<const uint32_t> rb = to_global(a.base + a.jobs[jd.job_r].base_off);
-    const gptr<const uint32_t> sb = to_global(a.base + a.jobs[jd.job_s].base_off);
+    const JobDesc &jr = a.jobs[jd.job_r], &js = a.jobs[jd.job_s];
     const gptr<unsigned long long> pt = to_global(a.ptask + jd.ptask_off);
     const gptr<unsigned long long> tmap = to_global(a.taskmap);
-    const uint32_t n = jd.nparts;  // <= 2^14; rb, sb hold n + 1 bases
+    const uint32_t n = jd.nparts;  // <= 2^14
     // One join: its task prefix is the whole queue, so the task -> partition
     // map and the queue totals are written here.
     const bool one = a.n_joins == 1;
     const unsigned long long map_cap = a.taskmap_cap;
+    // a join whose regions overflowed gets no tasks (it is run again with the exact layout)
+    const bool over = region_overflow(a, jr) || region_overflow(a, js);
+    if (over && threadIdx.x == 0) a.ctl->overflow = 1;
     const Spans sp(n);
-    // partition sizes: every base is loaded before the first division
+    // partition sizes: every size is loaded before the first division
     uint32_t nr[kScanRounds], ns[kScanRounds];
 #pragma unroll
     for (int j = 0; j < kScanRounds; j++) {
         const uint32_t i = sp.ok(j) ? sp.elem(j) : 0u;
-        nr[j] = rb[i + 1] - rb[i];
-        ns[j] = sb[i + 1] - sb[i];
+        uint32_t lo;
+        nr[j] = over ? 0u : part_rows(a, jr, i, &lo);
+        ns[j] = over ? 0u : part_rows(a, js, i, &lo);
     }
+    // region layout: the partitions' first and end rows for the build+probe kernel
+    auto put_ranges = [&](const JobDesc &job, const uint32_t (&cnt)[kScanRounds]) {
+        if (!job.region1) return;
+        const gptr<uint32_t> b0 = to_global(a.base + job.base_off), e0 = to_global(a.base + job.end_off);
+#pragma unroll
+        for (int j = 0; j < kScanRounds; j++)
+            if (sp.ok(j)) {
+                const uint32_t q = sp.elem(j);
+                b0[q] = q * job.region2;
+                e0[q] = q * job.region2 + cnt[j];
+            }
+    };
+    put_ranges(jr, nr);
+    put_ranges(js, ns);
     unsigned long long v[kScanRounds];
     const uint32_t lr = (uint32_t)__builtin_ctz(jd.cap_r), ls = (uint32_t)__builtin_ctz(jd.cap_s);
 #pragma unroll
@@ -266,6 +302,25 @@ __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
             a.ctl->task_total = total;
         }
     }
+}
+
+// Region layout, before pass 1: zeroes every job's region counters, the task
+// queue ticket, the overflow flag and the output counters (one block per job).
+__global__ __launch_bounds__(1024) void k_region_reset(PipeArgs a) {
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) {
+            a.ctl->task_ticket = 0;
+            a.ctl->overflow = 0;
+        }
+        for (uint32_t j = threadIdx.x; j < a.n_joins; j += blockDim.x) a.out_count[j] = 0;
+    }
+    if (blockIdx.x >= a.n_jobs) return;
+    const JobDesc &job = a.jobs[blockIdx.x];
+    if (!job.region1) return;
+    const gptr<uint32_t> m1 = to_global(a.counts + job.m1_off), m2 = to_global(a.counts + job.m2_off);
+    const uint32_t n1 = job.nseg << job.b1, nb = 1u << (job.b1 + job.b2);
+    for (uint32_t i = threadIdx.x; i < n1; i += blockDim.x) m1[i] = 0;
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) m2[i] = 0;
 }
 
 // Resets the queue and the output counters to run build+probe again (after
@@ -322,6 +377,16 @@ hipError_t launch_scan(const PipeArgs &a, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_bases, dim3(a.n_jobs ? a.n_jobs : 1), dim3(1024), 0, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    return launch_tasks(a, s);
+}
+
+hipError_t launch_region_reset(const PipeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_region_reset, dim3(a.n_jobs ? a.n_jobs : 1), dim3(1024), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_tasks(const PipeArgs &a, hipStream_t s) {
+    hipError_t e = hipSuccess;
     if (a.n_joins) {
         hipLaunchKernelGGL(k_tasks, dim3(a.n_joins), dim3(1024), 0, s, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -117,6 +117,10 @@ struct PassTile {
     uint32_t bins, shift;
     uint32_t job, local;    // job and tile (pass 1) / chunk (pass 2) index inside it
     uint32_t valid;
+    // region layout (region == 0: exact layout): digit d's counter is
+    // counts[rc0 + d * rstride], its region starts at row rrow0 + d * rdstride
+    uint32_t region, rstride, rrow0, rdstride, trash;
+    uint64_t rc0;
 };
 
 // Waves per SIMD the pass kernel is compiled for: 8 (64 VGPRs, four blocks per
@@ -154,15 +158,30 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>()))
         // groups of kXcdGroup consecutive chunks round-robin, so that a run of
         // heavy chunks (a skewed partition) is spread over all XCDs (block k
         // of XCD x takes chunk (k / G * 8 + x) * G + k % G).
+        // In the region layout the runs into one final region arrive in
+        // atomic order, so pass 2 keeps every chunk of a pass-1 digit d1 on
+        // XCD d1 % kXcds instead: a region's partial lines then merge in one
+        // L2. Each job's chunk count (2^b1 digits x c chunks) is a multiple of
+        // kXcds; XCD x takes job j's digits x, x + kXcds, ... in turn.
         const uint32_t *pfx = pass == 1 ? a.p1_tile0 : a.p2_chunk0;
         const uint32_t total = pass == 1 ? a.total_p1_tiles : a.total_p2_chunks;
         const uint32_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
-        const uint32_t b = pass == 1 ? x * ((total + kXcds - 1) / kXcds) + k
-                                     : ((k / kXcdGroup) * kXcds + x) * kXcdGroup + k % kXcdGroup;
-        const uint32_t valid = b < total;
-        const uint32_t j = valid ? wave_upper_index(pfx, a.n_jobs, b) : 0;
+        uint32_t b, valid, j;
+        if (pass == 2 && a.regions) {
+            valid = k * kXcds < total;
+            j = valid ? wave_upper_index(pfx, a.n_jobs, k * kXcds) : 0;
+            const JobDesc &jb = a.jobs[j];
+            const uint32_t c = jb.nseg * jb.region_tiles, kk = k - pfx[j] / kXcds;
+            b = valid ? pfx[j] + (x + kXcds * (kk / c)) * c + kk % c : total;
+        } else {
+            b = pass == 1 ? x * ((total + kXcds - 1) / kXcds) + k
+                          : ((k / kXcdGroup) * kXcds + x) * kXcdGroup + k % kXcdGroup;
+            valid = b < total;
+            j = valid ? wave_upper_index(pfx, a.n_jobs, b) : 0;
+        }
         if (lane == 0) {
             ti.valid = valid;
+            ti.region = 0;
             if (valid) {
                 const JobDesc &job = a.jobs[j];
                 const uint32_t local = b - pfx[j];
@@ -176,6 +195,39 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>()))
                     ti.hi = ti.lo + T < job.n ? ti.lo + T : job.n;
                     ti.bins = 1u << job.b1;
                     ti.shift = job.shift0;
+                    if (job.region1) {  // region (d1, g) of mid, g = this tile's segment
+                        const uint32_t g = local / job.seg_tiles;
+                        ti.region = job.region1;
+                        ti.rc0 = job.m1_off + g;
+                        ti.rstride = job.nseg;
+                        ti.rrow0 = g * job.region1;
+                        ti.rdstride = job.nseg * job.region1;
+                        ti.trash = (job.nseg << job.b1) * job.region1;
+                    }
+                } else if (job.region1) {
+                    // chunk = tile k of pass-1 region (d1, g); digit d2's region is (d1 << b2) | d2 of fin
+                    const uint32_t per = job.nseg * job.region_tiles;
+                    const uint32_t d1 = local / per, r = local - d1 * per;
+                    const uint32_t g = r / job.region_tiles, k = r - g * job.region_tiles;
+                    const uint32_t c1 = a.counts[job.m1_off + d1 * job.nseg + g];
+                    const uint32_t filled = c1 < job.region1 ? c1 : job.region1;
+                    const uint32_t start = (d1 * job.nseg + g) * job.region1;
+                    for (int c = 0; c < 4; c++) ti.src.cols[c] = job.mid_cols[c];
+                    ti.src.rowids = job.mid_ids;
+                    ti.src.n = job.n;
+                    for (int c = 0; c < 4; c++) ti.dst_cols[c] = job.fin_cols[c];
+                    ti.dst_ids = job.fin_ids;
+                    ti.lo = start + k * T;
+                    ti.hi = start + ((k + 1) * T < filled ? (k + 1) * T : filled);
+                    ti.valid = k * T < filled;
+                    ti.bins = 1u << job.b2;
+                    ti.shift = job.shift0 + job.b1;
+                    ti.region = job.region2;
+                    ti.rc0 = job.m2_off + (d1 << job.b2);
+                    ti.rstride = 1;
+                    ti.rrow0 = (d1 << job.b2) * job.region2;
+                    ti.rdstride = job.region2;
+                    ti.trash = (1u << (job.b1 + job.b2)) * job.region2;
                 } else {
                     const uint32_t n1 = 1u << job.b1;
                     const uint32_t d1 = local / job.nseg, g = local - d1 * job.nseg;
@@ -196,7 +248,7 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>()))
     }
     __syncthreads();
     if (!ti.valid) return;
-    if (tid < ti.bins) {  // first destination row of every digit
+    if (tid < ti.bins && !ti.region) {  // first destination row of every digit (exact layout)
         const JobDesc &job = a.jobs[ti.job];
         const uint32_t *base = a.base + job.base_off;
         if (pass == 1) {
@@ -271,7 +323,14 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>()))
         const uint32_t my_loc = block_excl_scan<uint32_t, kPassWaves>(cnt, sh_scan, &total);
         if (tid < bins) {
             loc[tid] = my_loc;
-            gdo[tid] = gdst[tid] - my_loc;  // output row of staged slot k = gdo[digit] + k
+            uint32_t dst;
+            if (ti.region) {  // reserve the run in the digit's region (a run that does not fit goes to the trash tile)
+                const uint32_t got = cnt ? atomicAdd(a.counts + ti.rc0 + (uint64_t)tid * ti.rstride, cnt) : 0u;
+                dst = (uint64_t)got + cnt <= ti.region ? ti.rrow0 + tid * ti.rdstride + got : ti.trash;
+            } else {
+                dst = gdst[tid];
+            }
+            gdo[tid] = dst - my_loc;  // output row of staged slot k = gdo[digit] + k
         }
         __syncthreads();
 
@@ -646,8 +705,8 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
                 if (lane == 0) {
                     const JobDesc &jr = a.jobs[jd.job_r];
                     const JobDesc &js = a.jobs[jd.job_s];
-                    const uint32_t *rb = a.base + jr.base_off, *sb = a.base + js.base_off;
-                    const uint32_t nr = rb[q + 1] - rb[q], ns = sb[q + 1] - sb[q];
+                    const uint32_t r0 = a.base[jr.base_off + q], s0 = a.base[js.base_off + q];
+                    const uint32_t nr = a.base[jr.end_off + q] - r0, ns = a.base[js.end_off + q] - s0;
                     const uint32_t s_chunks = (ns + jd.cap_s - 1) / jd.cap_s;
                     const uint32_t k = (uint32_t)(local - pt[q]);
                     const uint32_t rc = k / s_chunks, sc = k - rc * s_chunks;
@@ -663,9 +722,9 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
                     } else {
                         t.s = js.in;
                     }
-                    t.r_lo = rb[q] + rc * jd.cap_r;
+                    t.r_lo = r0 + rc * jd.cap_r;
                     t.r_n = nr - rc * jd.cap_r < jd.cap_r ? nr - rc * jd.cap_r : jd.cap_r;
-                    t.s_lo = sb[q] + sc * jd.cap_s;
+                    t.s_lo = s0 + sc * jd.cap_s;
                     t.s_n = ns - sc * jd.cap_s < jd.cap_s ? ns - sc * jd.cap_s : jd.cap_s;
                     uint32_t ns2 = 64;
                     // load <= 1/4 where the slot array (2 * CAP) allows, else <= 1/2:

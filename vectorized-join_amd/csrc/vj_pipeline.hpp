@@ -27,6 +27,20 @@
 // row-id column, once after pass 1 ("mid", only for 2-pass jobs) and once
 // after the last pass ("fin"). Partition q of a job occupies rows
 // [base[q], base[q+1]) of "fin"; q = d1 * 2^b2 + d2 (pass-1 digit major).
+//
+// Region layout (2-pass pipelines that ask for it): no histogram and no scans.
+// Every destination is a region of fixed capacity, and blocks reserve their
+// runs in it with one device atomic per digit:
+//  * pass 1 writes tile t's rows of digit d1 into region (d1, g) of "mid",
+//    g = t / seg_tiles (the segment: segments spread the atomics over
+//    2^b1 * nseg counters), rows [(d1 * nseg + g) * region1, + region1);
+//  * pass 2 reads that region in tiles (chunk = (d1, g, tile k)) and writes
+//    its rows of digit d2 into region q = (d1 << b2) | d2 of "fin", rows
+//    [q * region2, + region2), counter m2[q].
+// A run that does not fit its region goes to a one-tile trash area after the
+// regions, and the counter keeps the full count, so an overflow is visible
+// after the passes (k_tasks then schedules no task for the join and sets
+// Ctl::overflow; the host runs the join again with the exact layout).
 #pragma once
 #include <stdint.h>
 
@@ -102,10 +116,17 @@ struct JobDesc {
     uint32_t hsplit;          // histogram rows (blocks) per segment
     uint32_t hrow_tiles;      // pass-1 tiles per histogram row (ceil(seg_tiles / hsplit))
     uint32_t nrow;            // histogram rows = nseg * hsplit (= histogram blocks)
-    uint32_t pad_;
+    // Region layout (0: exact layout): rows per pass-1 region (d1, segment),
+    // rows per final region (partition), pass-2 chunks (tiles) per pass-1 region
+    uint32_t region1, region2, region_tiles;
+    // u32 offset of the partition ends in the base arena: partition q is rows
+    // [base[base_off + q], base[end_off + q]) of fin (exact layout: end_off =
+    // base_off + 1; region layout: a second array, both written by k_tasks)
+    uint32_t end_off;
+    uint32_t pad_[2];
     uint64_t tilepre_off;     // u32 offset of tilepre [p1_tiles][2^b1] in the count arena
-    uint64_t m1_off;          // u32 offset of m1 [nrow + 1][2^b1]
-    uint64_t m2_off;          // u32 offset of m2 [nrow + 1][2^(b1+b2)] (2-pass jobs)
+    uint64_t m1_off;          // u32 offset of m1 [nrow + 1][2^b1] (region layout: counters [2^b1][nseg])
+    uint64_t m2_off;          // u32 offset of m2 [nrow + 1][2^(b1+b2)] (2-pass jobs; region layout: counters [2^(b1+b2)])
     // Resident build sides. res_base != null: the job is already partitioned
     // (fin_* point at a resident table, res_base at its 2^(b1+b2)+1 bases) and
     // no hist / pass work is scheduled for it. keep_base != null: the bases
@@ -135,12 +156,16 @@ struct ColScanDesc {
     uint32_t pad;
 };
 
-// Control words: the ticket is reset by k_bases, the total written by
-// k_tasks (one join) or k_scan2, before the build+probe kernel reads them.
+// Control words: the ticket is reset by k_bases (k_region_reset in the region
+// layout), the total written by k_tasks (one join) or k_scan2, before the
+// build+probe kernel reads them. overflow: a region of the region layout
+// received more rows than it holds (set by k_tasks).
 struct Ctl {
     unsigned long long task_ticket;
     unsigned long long task_total;
+    unsigned long long overflow;
 };
+
 
 struct PipeArgs {
     const JobDesc *jobs;
@@ -163,8 +188,24 @@ struct PipeArgs {
     uint32_t total_hist_blocks, total_p1_tiles, total_p2_chunks;
     uint32_t max_hist_bins;         // LDS bins of the histogram kernel
     uint32_t pass_tile;             // tuples per radix-pass tile (pass_tile(key bytes, tile shape))
+    uint32_t regions;               // region layout (every job: two passes, 2^b1 a multiple of kXcds)
     HashCfg hash;
 };
+
+// Partition q of a job: its first row in "fin" (*lo) and its row count, from
+// the bases (exact layout) or the region counter (region layout, before
+// k_tasks has written the bases; a region that overflowed reports its
+// capacity, and the join is then run again).
+__device__ __forceinline__ uint32_t part_rows(const PipeArgs &a, const JobDesc &job, uint32_t q, uint32_t *lo) {
+    if (job.region2) {
+        const uint32_t c = a.counts[job.m2_off + q];
+        *lo = q * job.region2;
+        return c < job.region2 ? c : job.region2;
+    }
+    const uint32_t *b = a.base + job.base_off;
+    *lo = b[q];
+    return b[q + 1] - b[q];
+}
 
 // Kernel launchers (vj_join_kernels.hip); all asynchronous on `s`.
 struct LaunchShape {
@@ -187,6 +228,8 @@ extern template struct StageLaunch<HP_XOR_HW>;
 extern template struct StageLaunch<HP_GENERIC>;
 hipError_t launch_hist(const PipeArgs &a, const LaunchShape &sh, hipStream_t s);
 hipError_t launch_scan(const PipeArgs &a, hipStream_t s);
+hipError_t launch_region_reset(const PipeArgs &a, hipStream_t s);
+hipError_t launch_tasks(const PipeArgs &a, hipStream_t s);
 hipError_t launch_reset_probe(const PipeArgs &a, hipStream_t s);
 hipError_t launch_pass(const PipeArgs &a, const LaunchShape &sh, int pass, hipStream_t s);
 hipError_t launch_probe(const PipeArgs &a, const LaunchShape &sh, int blocks, hipStream_t s);

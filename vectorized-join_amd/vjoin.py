@@ -46,12 +46,15 @@ class HashOpts(ctypes.Structure):
 class JoinOpts(ctypes.Structure):
     _fields_ = [("hash", HashOpts), ("passes", ctypes.c_int32), ("bits1", ctypes.c_int32),
                 ("bits2", ctypes.c_int32), ("one_based", ctypes.c_int32), ("out_capacity_hint", ctypes.c_uint64),
-                ("tile", ctypes.c_int32), ("table", ctypes.c_int32)]
+                ("tile", ctypes.c_int32), ("table", ctypes.c_int32), ("layout", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 # Kernel shapes (vj_join_opts.tile / .table; 0 = planner / autotuned)
 TILE_STANDARD, TILE_SMALL = 1, 2
 TABLE_STANDARD, TABLE_WIDE = 1, 2
+# Partition layouts (vj_join_opts.layout; 0 = planner / autotuned)
+LAYOUT_EXACT, LAYOUT_REGIONS = 1, 2
 
 
 class Rows(ctypes.Structure):
@@ -88,14 +91,15 @@ class JoinSummary(ctypes.Structure):
                 ("retries", ctypes.c_int32), ("partitions", ctypes.c_uint64), ("tasks", ctypes.c_uint64),
                 ("ms_total", ctypes.c_double), ("ms_histogram", ctypes.c_double), ("ms_pass1", ctypes.c_double),
                 ("ms_pass2", ctypes.c_double), ("ms_build_probe", ctypes.c_double),
-                ("scratch_bytes", ctypes.c_uint64), ("graph_replay", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("scratch_bytes", ctypes.c_uint64), ("graph_replay", ctypes.c_int32),
+                ("layout_fallback", ctypes.c_int32),
                 ("bytes_histogram", ctypes.c_uint64), ("bytes_pass1", ctypes.c_uint64),
                 ("bytes_pass2", ctypes.c_uint64), ("bytes_build_probe", ctypes.c_uint64),
                 ("bytes_total", ctypes.c_uint64), ("gbps_histogram", ctypes.c_double),
                 ("gbps_pass1", ctypes.c_double), ("gbps_pass2", ctypes.c_double),
                 ("gbps_build_probe", ctypes.c_double), ("gbps_total", ctypes.c_double),
                 ("probe_tuples_per_s", ctypes.c_double), ("tile_rows", ctypes.c_int32),
-                ("probe_threads", ctypes.c_int32), ("table_rows", ctypes.c_int32), ("reserved2", ctypes.c_int32)]
+                ("probe_threads", ctypes.c_int32), ("table_rows", ctypes.c_int32), ("layout", ctypes.c_int32)]
 
 
 def lib():
@@ -214,9 +218,9 @@ def hash_table(cols, kind="fnv", hash_bits=64, chunk_bits=64, word_mode=WORDS_RE
 
 
 def join_opts(kind="fnv", hash_bits=64, chunk_bits=64, word_mode=WORDS_REFERENCE, passes=-1, bits1=0, bits2=0,
-              one_based=False, out_capacity_hint=0, tile=0, table=0):
+              one_based=False, out_capacity_hint=0, tile=0, table=0, layout=0):
     return JoinOpts(hash_opts(kind, hash_bits, chunk_bits, word_mode), passes, bits1, bits2, 1 if one_based else 0,
-                    out_capacity_hint, tile, table)
+                    out_capacity_hint, tile, table, layout, 0)
 
 
 class Context:
@@ -328,13 +332,13 @@ class Context:
                                   ctypes.c_void_p(offsets.data_ptr()), stream_ptr())
 
     def tune(self, build, probe, opts, reps=3):
-        """vj_tune: returns ((passes, bits1, bits2, tile, table), median ms) of the fastest plan, kept for the
-        size class (tile / table: TILE_* / TABLE_*)."""
+        """vj_tune: returns ((passes, bits1, bits2, tile, table, layout), median ms) of the fastest plan, kept
+        for the size class (tile / table / layout: TILE_* / TABLE_* / LAYOUT_*)."""
         best = JoinOpts()
         ms = ctypes.c_double()
         check(lib().vj_tune(self.ptr, ctypes.byref(build), ctypes.byref(probe), ctypes.byref(opts), int(reps),
                             ctypes.byref(best), ctypes.byref(ms), stream_ptr()), "vj_tune", self.ptr)
-        return (best.passes, best.bits1, best.bits2, best.tile, best.table), ms.value
+        return (best.passes, best.bits1, best.bits2, best.tile, best.table, best.layout), ms.value
 
     def tune_clear(self):
         check(lib().vj_tune_clear(self.ptr), "vj_tune_clear", self.ptr)

@@ -232,20 +232,10 @@ __device__ __forceinline__ unsigned long long part_tasks(uint32_t nr, uint32_t n
     return nr && ns ? (unsigned long long)(((nr - 1) >> lr) + 1) * (((ns - 1) >> ls) + 1) : 0ull;
 }
 
-// Whether any region of a region-layout job received more rows than it holds
-// (block-wide; false for exact-layout jobs).
-__device__ bool region_overflow(const PipeArgs &a, const JobDesc &job) {
-    if (!job.region1) return false;
-    const gptr<const uint32_t> m1 = to_global(a.counts + job.m1_off), m2 = to_global(a.counts + job.m2_off);
-    const uint32_t n1 = job.nseg << job.b1, nb = 1u << (job.b1 + job.b2);
-    int over = 0;
-    for (uint32_t i = threadIdx.x; i < n1; i += blockDim.x) over |= m1[i] > job.region1;
-    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) over |= m2[i] > job.region2;
-    return __syncthreads_or(over) != 0;
-}
-
 __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
     __shared__ unsigned long long sh64[20];
+    __shared__ unsigned long long rows[2];
+    __shared__ int bad;
     const uint32_t jn = blockIdx.x;
     const JoinDesc &jd = a.joins[jn];
     const JobDesc &jr = a.jobs[jd.job_r], &js = a.jobs[jd.job_s];
@@ -256,33 +246,72 @@ __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
     // map and the queue totals are written here.
     const bool one = a.n_joins == 1;
     const unsigned long long map_cap = a.taskmap_cap;
-    // a join whose regions overflowed gets no tasks (it is run again with the exact layout)
-    const bool over = region_overflow(a, jr) || region_overflow(a, js);
-    if (over && threadIdx.x == 0) a.ctl->overflow = 1;
     const Spans sp(n);
-    // partition sizes: every size is loaded before the first division
+    // partition sizes: every load issued before the first use
     uint32_t nr[kScanRounds], ns[kScanRounds];
+    if (!jr.region1) {  // exact layout (both jobs of a join share the layout): n + 1 bases per job
+        const gptr<const uint32_t> rb = to_global(a.base + jr.base_off), sb = to_global(a.base + js.base_off);
 #pragma unroll
-    for (int j = 0; j < kScanRounds; j++) {
-        const uint32_t i = sp.ok(j) ? sp.elem(j) : 0u;
-        uint32_t lo;
-        nr[j] = over ? 0u : part_rows(a, jr, i, &lo);
-        ns[j] = over ? 0u : part_rows(a, js, i, &lo);
-    }
-    // region layout: the partitions' first and end rows for the build+probe kernel
-    auto put_ranges = [&](const JobDesc &job, const uint32_t (&cnt)[kScanRounds]) {
-        if (!job.region1) return;
-        const gptr<uint32_t> b0 = to_global(a.base + job.base_off), e0 = to_global(a.base + job.end_off);
+        for (int j = 0; j < kScanRounds; j++) {
+            const uint32_t i = sp.ok(j) ? sp.elem(j) : 0u;
+            nr[j] = rb[i + 1] - rb[i];
+            ns[j] = sb[i + 1] - sb[i];
+        }
+    } else {
+        // region layout: the final region counters. The join overflowed when a
+        // counter passed its region, or when the counters miss rows (a run
+        // that did not fit its pass-1 region never reached pass 2); it then
+        // gets no tasks and is run again with the exact layout.
+        const gptr<const uint32_t> cr = to_global(a.counts + jr.m2_off), cs = to_global(a.counts + js.m2_off);
+#pragma unroll
+        for (int j = 0; j < kScanRounds; j++) {
+            const uint32_t i = sp.ok(j) ? sp.elem(j) : 0u;
+            nr[j] = cr[i];
+            ns[j] = cs[i];
+        }
+        if (threadIdx.x == 0) {
+            rows[0] = rows[1] = 0;
+            bad = 0;
+        }
+        __syncthreads();
+        unsigned long long sr = 0, ss = 0;
+        int over = 0;
+#pragma unroll
+        for (int j = 0; j < kScanRounds; j++) {
+            if (!sp.ok(j)) nr[j] = ns[j] = 0;
+            sr += nr[j];
+            ss += ns[j];
+            over |= nr[j] > jr.region2 || ns[j] > js.region2;
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            sr += __shfl_xor(sr, d, 64);
+            ss += __shfl_xor(ss, d, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&rows[0], sr);
+            atomicAdd(&rows[1], ss);
+        }
+        if (over) bad = 1;
+        __syncthreads();
+        if (bad || rows[0] != jr.n || rows[1] != js.n) {
+            if (threadIdx.x == 0) a.ctl->overflow = 1;
+#pragma unroll
+            for (int j = 0; j < kScanRounds; j++) nr[j] = ns[j] = 0;
+        }
+        // the partitions' first and end rows for the build+probe kernel
+        const gptr<uint32_t> rb0 = to_global(a.base + jr.base_off), re0 = to_global(a.base + jr.end_off);
+        const gptr<uint32_t> sb0 = to_global(a.base + js.base_off), se0 = to_global(a.base + js.end_off);
 #pragma unroll
         for (int j = 0; j < kScanRounds; j++)
             if (sp.ok(j)) {
                 const uint32_t q = sp.elem(j);
-                b0[q] = q * job.region2;
-                e0[q] = q * job.region2 + cnt[j];
+                rb0[q] = q * jr.region2;
+                re0[q] = q * jr.region2 + nr[j];
+                sb0[q] = q * js.region2;
+                se0[q] = q * js.region2 + ns[j];
             }
-    };
-    put_ranges(jr, nr);
-    put_ranges(js, ns);
+    }
     unsigned long long v[kScanRounds];
     const uint32_t lr = (uint32_t)__builtin_ctz(jd.cap_r), ls = (uint32_t)__builtin_ctz(jd.cap_s);
 #pragma unroll

@@ -192,21 +192,6 @@ struct PipeArgs {
     HashCfg hash;
 };
 
-// Partition q of a job: its first row in "fin" (*lo) and its row count, from
-// the bases (exact layout) or the region counter (region layout, before
-// k_tasks has written the bases; a region that overflowed reports its
-// capacity, and the join is then run again).
-__device__ __forceinline__ uint32_t part_rows(const PipeArgs &a, const JobDesc &job, uint32_t q, uint32_t *lo) {
-    if (job.region2) {
-        const uint32_t c = a.counts[job.m2_off + q];
-        *lo = q * job.region2;
-        return c < job.region2 ? c : job.region2;
-    }
-    const uint32_t *b = a.base + job.base_off;
-    *lo = b[q];
-    return b[q + 1] - b[q];
-}
-
 // Kernel launchers (vj_join_kernels.hip); all asynchronous on `s`.
 struct LaunchShape {
     int hp, kw, nc;  // hash path, key width, key columns

@@ -43,11 +43,11 @@ def algorithmic_bytes(nr, ns, m, passes):
     return (2 * passes + 1) * (nr + ns) * w + m * PAIR_BYTES
 
 
-def stage_bytes(nr, ns, m):
-    """Bytes each stage must move (its algorithmic traffic)."""
+def stage_bytes(nr, ns, m, regions=False):
+    """Bytes each stage must move (its algorithmic traffic; the region layout has no histogram read)."""
     n = nr + ns
     return {
-        "histogram": n * KEY_BYTES,                                   # read keys
+        "histogram": 0 if regions else n * KEY_BYTES,                 # read keys
         "pass1": n * (KEY_BYTES + KEY_BYTES + ROWID_BYTES),            # read keys, write key + row id
         "pass2": n * 2 * (KEY_BYTES + ROWID_BYTES),                   # read + write key + row id
         "build_probe": n * (KEY_BYTES + ROWID_BYTES) + m * PAIR_BYTES,  # read tuples, write pairs
@@ -954,7 +954,9 @@ def fk_leg(ctx, args, workload_tag, keep_host):
     summ = ctx.summary()
     ms_step = elapsed / args.steps * 1e3
     med = {k: float(np.median(v)) for k, v in stages.items()}
-    sb = stage_bytes(nr, ns, m)
+    regions = summ["layout"] == vjoin.LAYOUT_REGIONS
+    layout_name = "regions" if regions else "exact"
+    sb = stage_bytes(nr, ns, m, regions)
     dominant = max(("histogram", "pass1", "pass2", "build_probe"), key=lambda k: med[k])
     achieved = sb[dominant] / (float(np.mean(stages[dominant])) * 1e-3)
     B = algorithmic_bytes(nr, ns, m, summ["passes"])
@@ -973,14 +975,15 @@ def fk_leg(ctx, args, workload_tag, keep_host):
         "dtype": "int64",
         "data": "synthetic (seeded: workloads.large_fk)",
         "config": {"workload": "large_fk: build 2^%d unique int64 keys (mix64 of a permutation), probe 2^%d FK "
-                               "keys (100%% match), FNV-1a-64, %d radix passes, bits %s" % (
+                               "keys (100%% match), FNV-1a-64, %d radix passes, bits and layout %s" % (
                                   args.log_r, args.log_s, summ["passes"],
                                   "of the planner" if args.no_tune else "autotuned"),
                    "build_rows": nr, "probe_rows": ns, "matches": m, "passes": summ["passes"],
-                   "bits": [summ["bits1"], summ["bits2"]], "parallelism": "single GPU"},
+                   "bits": [summ["bits1"], summ["bits2"]], "layout": layout_name, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "%s (%s)" % (STAGE_KERNEL[dominant], dominant),
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": read_traffic(dominant, workload_tag),
+                     "frac": achieved / HBM_PEAK,
+                     "traffic": read_traffic(dominant, "%s layout %s" % (workload_tag, layout_name)),
                      "algorithmic_bytes_per_launch": sb[dominant]},
         "join_roofline": {"algorithmic_bytes": B, "ms": med["total"], "achieved_GBps": join_gbs / 1e9,
                           "frac": join_gbs / HBM_PEAK},

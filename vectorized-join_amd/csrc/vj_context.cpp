@@ -598,6 +598,8 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
         jd.nparts = 1u << (P >= 1 ? b1 : 0) << (P >= 2 ? b2 : 0);
         jd.ptask_off = (uint32_t)pl.ptask_words;
         pl.ptask_words += jd.nparts + 1;
+        jd.part_off = (uint32_t)pl.ptask_words;  // the task-prefix blocks' partials
+        pl.ptask_words += 4ull * ((jd.nparts + kTaskBlock - 1) / kTaskBlock);
         pl.taskmap_cap += 2ull * jd.nparts;
         pl.max_nparts = std::max(pl.max_nparts, jd.nparts);
         jd.slot_shift = (P >= 1 ? b1 : 0) + (P >= 2 ? b2 : 0);

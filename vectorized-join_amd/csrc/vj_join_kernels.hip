@@ -6,9 +6,10 @@
 //   k_colscan   exclusive scans down the segment columns of those counts
 //   k_bases     per job: partition bases from the column totals; block 0
 //               also resets the task queue and the output counters
-//   k_tasks     per join: build/probe task prefix over partition pairs (one
-//               join: also the queue totals and the task map); in the region
-//               layout also the overflow check
+//   k_tasks, k_task_offsets  per join, one block per 1024 partitions:
+//               build/probe task prefix over partition pairs (one join: also
+//               the queue totals and the task map); in the region layout also
+//               the overflow check
 //   k_region_reset  region layout: zeroes the region counters (replaces
 //               k_hist, k_colscan and k_bases)
 //   k_scan2     task prefix across joins                 (1 block, >1 join)
@@ -232,103 +233,127 @@ __device__ __forceinline__ unsigned long long part_tasks(uint32_t nr, uint32_t n
     return nr && ns ? (unsigned long long)(((nr - 1) >> lr) + 1) * (((ns - 1) >> ls) + 1) : 0ull;
 }
 
+// Task prefix, first kernel: one block per kTaskBlock partitions of a join.
+// Per partition the task count (R chunks x S chunks), its exclusive prefix
+// inside the block (into ptask), and per block the total and, in the region
+// layout, the build and probe row sums and whether a counter passed its
+// region (the block's partials); the region layout's partition first / end
+// rows are written here too.
 __global__ __launch_bounds__(1024) void k_tasks(PipeArgs a) {
-    __shared__ unsigned long long sh64[20];
+    __shared__ unsigned long long sh64[kTaskBlock / 64];
     __shared__ unsigned long long rows[2];
     __shared__ int bad;
-    const uint32_t jn = blockIdx.x;
+    const uint32_t jn = blockIdx.x, blk = blockIdx.y, tid = threadIdx.x;
     const JoinDesc &jd = a.joins[jn];
+    const uint32_t n = jd.nparts;
+    if (blk * kTaskBlock >= n) return;  // block-uniform
     const JobDesc &jr = a.jobs[jd.job_r], &js = a.jobs[jd.job_s];
-    const gptr<unsigned long long> pt = to_global(a.ptask + jd.ptask_off);
-    const gptr<unsigned long long> tmap = to_global(a.taskmap);
-    const uint32_t n = jd.nparts;  // <= 2^14
-    // One join: its task prefix is the whole queue, so the task -> partition
-    // map and the queue totals are written here.
-    const bool one = a.n_joins == 1;
-    const unsigned long long map_cap = a.taskmap_cap;
-    const Spans sp(n);
-    // partition sizes: every load issued before the first use
-    uint32_t nr[kScanRounds], ns[kScanRounds];
-    if (!jr.region1) {  // exact layout (both jobs of a join share the layout): n + 1 bases per job
+    const uint32_t q = blk * kTaskBlock + tid;
+    const bool ok = q < n;
+    const uint32_t qi = ok ? q : 0u;
+    const bool regions = jr.region1 != 0;  // both jobs of a join share the layout
+    uint32_t nr, ns;
+    if (!regions) {  // exact layout: n + 1 bases per job
         const gptr<const uint32_t> rb = to_global(a.base + jr.base_off), sb = to_global(a.base + js.base_off);
-#pragma unroll
-        for (int j = 0; j < kScanRounds; j++) {
-            const uint32_t i = sp.ok(j) ? sp.elem(j) : 0u;
-            nr[j] = rb[i + 1] - rb[i];
-            ns[j] = sb[i + 1] - sb[i];
-        }
-    } else {
-        // region layout: the final region counters. The join overflowed when a
-        // counter passed its region, or when the counters miss rows (a run
-        // that did not fit its pass-1 region never reached pass 2); it then
-        // gets no tasks and is run again with the exact layout.
-        const gptr<const uint32_t> cr = to_global(a.counts + jr.m2_off), cs = to_global(a.counts + js.m2_off);
-#pragma unroll
-        for (int j = 0; j < kScanRounds; j++) {
-            const uint32_t i = sp.ok(j) ? sp.elem(j) : 0u;
-            nr[j] = cr[i];
-            ns[j] = cs[i];
-        }
-        if (threadIdx.x == 0) {
+        const uint32_t r0 = rb[qi], r1 = rb[qi + 1], s0 = sb[qi], s1 = sb[qi + 1];
+        nr = r1 - r0;
+        ns = s1 - s0;
+    } else {  // region layout: the final region counters
+        nr = to_global(a.counts + jr.m2_off)[qi];
+        ns = to_global(a.counts + js.m2_off)[qi];
+    }
+    if (!ok) nr = ns = 0;
+    const gptr<unsigned long long> part = to_global(a.ptask + jd.part_off + (uint64_t)blk * 4);
+    if (regions) {
+        if (tid == 0) {
             rows[0] = rows[1] = 0;
             bad = 0;
         }
         __syncthreads();
-        unsigned long long sr = 0, ss = 0;
-        int over = 0;
-#pragma unroll
-        for (int j = 0; j < kScanRounds; j++) {
-            if (!sp.ok(j)) nr[j] = ns[j] = 0;
-            sr += nr[j];
-            ss += ns[j];
-            over |= nr[j] > jr.region2 || ns[j] > js.region2;
-        }
+        unsigned long long sr = nr, ss = ns;
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
             sr += __shfl_xor(sr, d, 64);
             ss += __shfl_xor(ss, d, 64);
         }
-        if ((threadIdx.x & 63) == 0) {
+        if ((tid & 63) == 0) {
             atomicAdd(&rows[0], sr);
             atomicAdd(&rows[1], ss);
         }
-        if (over) bad = 1;
+        if (nr > jr.region2 || ns > js.region2) bad = 1;
         __syncthreads();
-        if (bad || rows[0] != jr.n || rows[1] != js.n) {
-            if (threadIdx.x == 0) a.ctl->overflow = 1;
-#pragma unroll
-            for (int j = 0; j < kScanRounds; j++) nr[j] = ns[j] = 0;
+        if (tid == 0) {
+            part[1] = rows[0];
+            part[2] = rows[1];
+            part[3] = (unsigned long long)bad;
         }
-        // the partitions' first and end rows for the build+probe kernel
-        const gptr<uint32_t> rb0 = to_global(a.base + jr.base_off), re0 = to_global(a.base + jr.end_off);
-        const gptr<uint32_t> sb0 = to_global(a.base + js.base_off), se0 = to_global(a.base + js.end_off);
-#pragma unroll
-        for (int j = 0; j < kScanRounds; j++)
-            if (sp.ok(j)) {
-                const uint32_t q = sp.elem(j);
-                rb0[q] = q * jr.region2;
-                re0[q] = q * jr.region2 + nr[j];
-                sb0[q] = q * js.region2;
-                se0[q] = q * js.region2 + ns[j];
-            }
+        nr = nr < jr.region2 ? nr : jr.region2;
+        ns = ns < js.region2 ? ns : js.region2;
+        if (ok) {
+            to_global(a.base + jr.base_off)[q] = q * jr.region2;
+            to_global(a.base + jr.end_off)[q] = q * jr.region2 + nr;
+            to_global(a.base + js.base_off)[q] = q * js.region2;
+            to_global(a.base + js.end_off)[q] = q * js.region2 + ns;
+        }
     }
-    unsigned long long v[kScanRounds];
     const uint32_t lr = (uint32_t)__builtin_ctz(jd.cap_r), ls = (uint32_t)__builtin_ctz(jd.cap_s);
-#pragma unroll
-    for (int j = 0; j < kScanRounds; j++) v[j] = sp.ok(j) ? part_tasks(nr[j], ns[j], lr, ls) : 0ull;
-    const unsigned long long total =
-        block_scan_spans<unsigned long long>(sp, v, sh64, [&](uint32_t e, unsigned long long pre, unsigned long long t) {
-            pt[e] = pre;
-            if (one)
-                for (unsigned long long q = pre; q < pre + t && q < map_cap; q++) tmap[q] = e;
-        });
-    if (threadIdx.x == 0) {
-        pt[n] = total;
-        a.join_tasks[jn] = total;
+    const unsigned long long t = ok ? part_tasks(nr, ns, lr, ls) : 0ull;
+    unsigned long long total;
+    const unsigned long long pre = block_excl_scan<unsigned long long, kTaskBlock / 64>(t, sh64, &total);
+    if (ok) to_global(a.ptask + jd.ptask_off)[q] = pre;
+    if (tid == 0) part[0] = total;
+}
+
+// Task prefix, second kernel (same grid): adds the totals of the blocks
+// before each block, writes the join's task total (one join: also the queue
+// totals and the task -> partition map). In the region layout the join
+// overflowed when a counter passed its region or the counters miss rows (a
+// run that did not fit its pass-1 region never reached pass 2); it then gets
+// no tasks, Ctl::overflow is set, and the host runs it again with the exact
+// layout.
+__global__ __launch_bounds__(1024) void k_task_offsets(PipeArgs a) {
+    const uint32_t jn = blockIdx.x, blk = blockIdx.y, tid = threadIdx.x;
+    const JoinDesc &jd = a.joins[jn];
+    const uint32_t n = jd.nparts, nblk = (n + kTaskBlock - 1) / kTaskBlock;
+    if (blk >= nblk) return;  // block-uniform
+    const JobDesc &jr = a.jobs[jd.job_r], &js = a.jobs[jd.job_s];
+    const gptr<const unsigned long long> part = to_global(a.ptask + jd.part_off);
+    unsigned long long off = 0, total = 0, sr = 0, ss = 0, bad = 0;
+    for (uint32_t b = 0; b < nblk; b++) {  // at most 16 blocks' partials
+        const unsigned long long x = part[b * 4];
+        off += b < blk ? x : 0ull;
+        total += x;
+        sr += part[b * 4 + 1];
+        ss += part[b * 4 + 2];
+        bad |= part[b * 4 + 3];
+    }
+    const bool over = jr.region1 != 0 && (bad != 0 || sr != jr.n || ss != js.n);
+    const bool one = a.n_joins == 1;
+    const gptr<unsigned long long> pt = to_global(a.ptask + jd.ptask_off);
+    const uint32_t q = blk * kTaskBlock + tid;
+    // this partition's in-block prefix and the next one's (the block total after the last)
+    unsigned long long pre = 0, next = 0;
+    if (q < n) {
+        pre = pt[q];
+        next = tid + 1 < kTaskBlock && q + 1 < n ? pt[q + 1] : part[blk * 4];
+    }
+    __syncthreads();  // every in-block prefix is read before any is rewritten
+    if (q < n) {
+        pt[q] = over ? 0ull : pre + off;
+        if (one && !over) {
+            const gptr<unsigned long long> tmap = to_global(a.taskmap);
+            for (unsigned long long k = pre + off; k < next + off && k < a.taskmap_cap; k++) tmap[k] = q;
+        }
+    }
+    if (blk == 0 && tid == 0) {
+        const unsigned long long tasks = over ? 0ull : total;
+        pt[n] = tasks;
+        a.join_tasks[jn] = tasks;
+        if (over) a.ctl->overflow = 1;
         if (one) {
             a.task0[0] = 0;
-            a.task0[1] = total;
-            a.ctl->task_total = total;
+            a.task0[1] = tasks;
+            a.ctl->task_total = tasks;
         }
     }
 }
@@ -417,7 +442,10 @@ hipError_t launch_region_reset(const PipeArgs &a, hipStream_t s) {
 hipError_t launch_tasks(const PipeArgs &a, hipStream_t s) {
     hipError_t e = hipSuccess;
     if (a.n_joins) {
-        hipLaunchKernelGGL(k_tasks, dim3(a.n_joins), dim3(1024), 0, s, a);
+        const dim3 grid(a.n_joins, (a.max_nparts + kTaskBlock - 1) / kTaskBlock);
+        hipLaunchKernelGGL(k_tasks, grid, dim3(kTaskBlock), 0, s, a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_task_offsets, grid, dim3(kTaskBlock), 0, s, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (a.n_joins == 1) return hipSuccess;  // k_tasks wrote the queue totals and the task map

@@ -144,7 +144,11 @@ struct JoinDesc {
     uint32_t nparts;          // 2^(b1+b2)
     uint32_t slot_shift;      // hash bits consumed by partitioning (slot uses h >> slot_shift)
     uint32_t one_based;
+    uint32_t part_off;        // u64 offset of the task-prefix blocks' partials [ceil(nparts / kTaskBlock)][4]
+    uint32_t pad_;
 };
+
+constexpr uint32_t kTaskBlock = 1024;  // partitions per block of the task-prefix kernels
 
 constexpr int kColScanCols = 64;     // columns per block of the column-scan kernel
 

@@ -145,18 +145,21 @@ typedef struct vj_join_opts {
     int32_t bits2;      /* radix bits of pass 2 (0 = planner; 1..8) */
     int32_t one_based;  /* emit row ids + 1 (the reference's primary_keys = row + 1) */
     uint64_t out_capacity_hint; /* expected pair count for vj_join (0 = probe rows) */
-    int32_t tile;       /* radix-pass tile: 0 = planner / autotuned, VJ_TILE_STANDARD or VJ_TILE_SMALL */
+    int32_t tile;       /* radix-pass tile: 0 = planner / autotuned, VJ_TILE_STANDARD, VJ_TILE_SMALL or VJ_TILE_LARGE */
     int32_t table;      /* build+probe block: 0 = planner / autotuned, VJ_TABLE_STANDARD or VJ_TABLE_WIDE */
     int32_t layout;     /* partition layout: 0 = planner / autotuned, VJ_LAYOUT_EXACT or VJ_LAYOUT_REGIONS */
     int32_t reserved;   /* 0 */
 } vj_join_opts;
 
 /* Kernel shapes (vj_join_opts.tile / .table). All joins of one pipeline (a
- * batch) share them. The planner takes the standard tile, and the wide table
+ * batch) share them. The planner takes the standard tile (the large one in
+ * the region layout), and the wide table
  * when the standard one would need more than 14 radix bits (build sides over
  * 2^24 rows of 8-byte keys); vj_tune times the alternatives. */
 #define VJ_TILE_STANDARD 1   /* 512 threads x 6 rows (keys <= 8 bytes), x 4 (<= 16), x 2 (wider) */
 #define VJ_TILE_SMALL 2      /* 512 threads x 2 rows: more blocks for small tables */
+#define VJ_TILE_LARGE 3      /* 512 threads x 8 rows (keys <= 8 bytes), x 6 (<= 16), x 4 (wider): longer runs
+                                per digit; the planner's tile in the region layout */
 #define VJ_TABLE_STANDARD 1  /* 512 threads (8 waves), up to four blocks per CU, LDS table of 2048 rows
                                 (keys <= 8 bytes) or 1024 rows */
 #define VJ_TABLE_WIDE 2      /* 1024 threads (16 waves), one block per CU, LDS table of 8192 rows
@@ -339,7 +342,7 @@ vj_status vj_join_batch(vj_ctx *ctx, int32_t n_joins, const vj_table *builds, co
  * planner's choice for each table shape (each pass count; radix bits -1..+1;
  * both splits of two passes, each in both partition layouts; standard and wide
  * build+probe blocks), then the
- * small pass tile with the fastest, on the caller's tables, `reps` timed runs
+ * other pass tiles with the fastest, on the caller's tables, `reps` timed runs
  * each after a warm-up, and keeps the fastest in the context for the size
  * class of `build` (build rows rounded up to a power of two, key bytes, hash
  * path). Later joins on this context with opts->passes == -1 and no bits set

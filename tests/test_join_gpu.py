@@ -146,7 +146,7 @@ SHAPE_CASES = [(4, 1, "fnv", 1, 3, 0, 40000), (8, 1, "fnv", 2, 3, 2, 150000), (4
                (4, 2, "fnv", -1, 0, 0, 70000)]
 
 
-@pytest.mark.parametrize("tile,table", [(1, 1), (2, 1), (1, 2), (2, 2)])
+@pytest.mark.parametrize("tile,table", [(1, 1), (2, 1), (1, 2), (2, 2), (3, 1), (3, 2)])
 @pytest.mark.parametrize("case", SHAPE_CASES)
 def test_kernel_shapes(ctx, tile, table, case):
     """Every pass tile and build+probe block shape gives the same pair set as the
@@ -163,7 +163,8 @@ def test_kernel_shapes(ctx, tile, table, case):
     got = _pairs(*_run(ctx, rc, sc, kind=kind, passes=passes, bits1=b1, bits2=b2, tile=tile, table=table))
     sm = ctx.summary()
     kb = kw * ncols
-    assert sm["tile_rows"] == 512 * (2 if tile == 2 else (6 if kb <= 8 else (4 if kb <= 16 else 2)))
+    rows = {1: (6, 4, 2), 2: (2, 2, 2), 3: (8, 6, 4)}[tile]
+    assert sm["tile_rows"] == 512 * (rows[0] if kb <= 8 else (rows[1] if kb <= 16 else rows[2]))
     assert sm["probe_threads"] == (1024 if table == 2 else 512)
     assert sm["table_rows"] == ({4: 8192, 8: 8192, 16: 4096, 24: 4096}[kb] if table == 2 else (2048 if kb <= 8 else 1024))
     want = sorted_pairs(*oracle.join(workloads.reference_rows(rc), workloads.reference_rows(sc), kind, 64, 64,
@@ -212,7 +213,7 @@ def test_shapes_in_batches_and_resident_tables(ctx):
 def test_bad_shapes_are_rejected(ctx):
     import torch
     d = torch.arange(100, dtype=torch.int32, device="cuda")
-    for tile, table in ((3, 0), (0, 3), (-1, 0)):
+    for tile, table in ((4, 0), (0, 3), (-1, 0)):
         out = torch.empty(100, dtype=torch.int32, device="cuda")
         rc, _ = ctx.join_into(vjoin.table([d]), vjoin.table([d]), vjoin.join_opts(tile=tile, table=table), out, out)
         assert rc == vjoin.STATUS_CODE["VJ_ERR_INVALID_ARGUMENT"], (tile, table, rc)
@@ -409,14 +410,15 @@ def test_tune_keeps_fastest_plan_and_results():
         (P, b1, b2, tile, table, layout), ms = c.tune(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts(), reps=2)
         assert ms > 0 and P in (0, 1, 2)
         assert (P == 0 and b1 == 0) or (P >= 1 and 1 <= b1 <= 8) and (P < 2 or 1 <= b2 <= 8)
-        assert tile in (vjoin.TILE_STANDARD, vjoin.TILE_SMALL) and table in (vjoin.TABLE_STANDARD, vjoin.TABLE_WIDE)
+        assert tile in (vjoin.TILE_STANDARD, vjoin.TILE_SMALL, vjoin.TILE_LARGE)
+        assert table in (vjoin.TABLE_STANDARD, vjoin.TABLE_WIDE)
         assert layout == vjoin.LAYOUT_EXACT or (P == 2 and layout == vjoin.LAYOUT_REGIONS)
         r, s_ = c.join(vjoin.table(dr), vjoin.table(ds), vjoin.join_opts())
         sm = c.summary()
         assert (sm["passes"], sm["bits1"], sm["bits2"] if P == 2 else 0) == (P, b1, b2 if P == 2 else 0)
         assert sm["layout"] == layout and sm["layout_fallback"] == 0
         # the tuned kernel shapes are used too (4-byte keys: tiles of 3072 or 1024 rows, tables of 2048 or 8192)
-        assert sm["tile_rows"] == (3072 if tile == vjoin.TILE_STANDARD else 1024)
+        assert sm["tile_rows"] == {vjoin.TILE_STANDARD: 3072, vjoin.TILE_SMALL: 1024, vjoin.TILE_LARGE: 4096}[tile]
         assert (sm["probe_threads"], sm["table_rows"]) == ((512, 2048) if table == vjoin.TABLE_STANDARD else (1024, 8192))
         _assert_same(_pairs(r, s_), want, "tuned")
         c.tune_clear()

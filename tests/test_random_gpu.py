@@ -1,7 +1,7 @@
 """Seeded random join configurations through the C ABI, each against the
 numpy join (whose result depends on the keys alone): key width and column
 count, hash kind, HASH_BITS and chunk width, pass count and radix bits, pass
-tile and build+probe block, partition layout (two-pass plans in the region
+tile (standard, small, large) and build+probe block, partition layout (two-pass plans in the region
 layout overflow and fall back where duplicates are dense), duplicate density,
 sizes from a few rows to 2^18, explicit row ids, 1-based output, signed keys,
 and the entry points (vj_join, vj_join_into with an exact or a larger capacity,
@@ -46,7 +46,7 @@ def _case(seed):
                 b1=b1, b2=b2, nr=nr, ns=ns, dens=distinct_per_row, tile=int(rng.integers(0, 3)),
                 table=int(rng.integers(0, 3)), signed=bool(rng.random() < 0.2), rowids=bool(rng.random() < 0.3),
                 one_based=bool(rng.random() < 0.3), entry=str(rng.choice(["join", "into", "into_big", "batch"])),
-                layout=int(rng.integers(0, 3)))
+                layout=int(rng.integers(0, 3)), large_tile=bool(rng.random() < 0.25))
 
 
 def _data(p, rng):
@@ -88,8 +88,9 @@ def test_random_configuration_matches_numpy_join(ctx, seed):
     dsr = None if sr is None else torch.from_numpy(sr.view(np.int32)).cuda()
     bt = vjoin.table(dr, drr, p["signed"])
     pt = vjoin.table(ds, dsr, p["signed"])
+    tile = vjoin.TILE_LARGE if p["large_tile"] else p["tile"]
     opts = vjoin.join_opts(p["kind"], p["hash_bits"], p["chunk_bits"], passes=p["passes"], bits1=p["b1"],
-                           bits2=p["b2"], one_based=p["one_based"], tile=p["tile"], table=p["table"],
+                           bits2=p["b2"], one_based=p["one_based"], tile=tile, table=p["table"],
                            layout=p["layout"])
     m = len(want[0])
     if p["entry"] == "join":

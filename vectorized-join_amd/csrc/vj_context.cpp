@@ -446,8 +446,9 @@ uint64_t tuned_word(int P, int b1, int b2, int tile, int table, int layout) {
 }
 
 Err check_shapes(const vj_join_opts *o) {
-    if (o->tile < 0 || o->tile > VJ_TILE_SMALL || o->table < 0 || o->table > VJ_TABLE_WIDE)
-        return Err::make(VJ_ERR_INVALID_ARGUMENT, "tile and table must be 0 (planner), 1 (standard) or 2");
+    if (o->tile < 0 || o->tile > VJ_TILE_LARGE || o->table < 0 || o->table > VJ_TABLE_WIDE)
+        return Err::make(VJ_ERR_INVALID_ARGUMENT,
+                         "tile must be 0 (planner), 1 (standard), 2 (small) or 3 (large); table 0, 1 or 2");
     if (o->layout < 0 || o->layout > VJ_LAYOUT_REGIONS)
         return Err::make(VJ_ERR_INVALID_ARGUMENT, "layout must be 0 (planner), 1 (exact) or 2 (regions)");
     return Err::ok();
@@ -462,9 +463,10 @@ uint64_t largest_build(const std::vector<JoinSpec> &specs) {
 
 // Kernel shapes of a pipeline (one for all of its joins): the caller's, else
 // the autotuned ones of the size class of the batch's largest build, else
-// the standard tile, and the wide table when a standard one would need more
-// than kMaxHistBits bits of partitioning to bring the largest build down to
-// half a chunk per partition.
+// the wide table when a standard one would need more than kMaxHistBits bits
+// of partitioning to bring the largest build down to half a chunk per
+// partition; a tile left open (-1) becomes the large tile in the region
+// layout and the standard one otherwise (build_join_plan).
 Err choose_shapes(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_opts *o, const vj_ctx *c) {
     VJ_TRY(check_shapes(o));
     const uint64_t nr = largest_build(specs);
@@ -476,7 +478,6 @@ Err choose_shapes(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_op
             if (table < 0) table = (int)((it->second >> 28) & 0xF);
         }
     }
-    if (tile < 0) tile = 0;
     if (table < 0) table = need_bits(nr, build_cap(pl.kb)) > kMaxHistBits ? 1 : 0;
     pl.shape.tile = tile;
     pl.shape.table = table;
@@ -500,7 +501,7 @@ bool choose_regions(const Plan &pl, const std::vector<JoinSpec> &specs, const st
         if (it != c->tuned.end() && !c->region_overflowed.count(key)) layout = (int)((it->second >> 32) & 0xF);
     }
     if (layout != VJ_LAYOUT_REGIONS || !allow) return false;
-    const uint32_t T = pass_tile(pl.kb, pl.shape.tile);
+    const uint32_t T = pass_tile(pl.kb, pl.shape.tile < 0 ? 2 : pl.shape.tile);
     for (size_t i = 0; i < specs.size(); i++) {
         const JoinSpec &js = specs[i];
         // pass 2 deals a job's digits to the XCDs: 2^b1 must be a multiple of kXcds
@@ -562,6 +563,7 @@ Err build_join_plan(Plan &pl, const std::vector<JoinSpec> &specs, const vj_join_
         }
     }
     pl.regions = choose_regions(pl, specs, bits, o, c, allow_regions);
+    if (pl.shape.tile < 0) pl.shape.tile = pl.regions ? 2 : 0;
     for (size_t i = 0; i < specs.size(); i++) {
         const JoinSpec &js = specs[i];
         const int P = bits[i][0], b1 = bits[i][1], b2 = bits[i][2];
@@ -1622,7 +1624,8 @@ vj_status vj_tune(vj_ctx *c, const vj_table *build, const vj_table *probe, const
         if (opts->layout && x.layout != opts->layout) return;
         cands.push_back(x);
     };
-    const int tile0 = opts->tile ? opts->tile - 1 : 0;
+    // the caller's tile, else the standard tile (exact layout) or the large one (region layout)
+    const int tile0 = opts->tile ? opts->tile - 1 : 0, tile_r = opts->tile ? opts->tile - 1 : 2;
     for (int table = 0; table <= 1; table++) {
         if (opts->table && table != opts->table - 1) continue;
         const int cap = probe_cap(kb, table);
@@ -1635,7 +1638,7 @@ vj_status vj_tune(vj_ctx *c, const vj_table *build, const vj_table *probe, const
                 const int b2 = tot - b1;
                 if (b1 >= 1 && b1 <= kMaxPassBits && b2 >= 1 && b2 <= kMaxPassBits) {
                     add({2, b1, b2, tile0, table, VJ_LAYOUT_EXACT});
-                    if ((1u << b1) % kXcds == 0) add({2, b1, b2, tile0, table, VJ_LAYOUT_REGIONS});
+                    if ((1u << b1) % kXcds == 0) add({2, b1, b2, tile_r, table, VJ_LAYOUT_REGIONS});
                 }
             }
     }
@@ -1691,14 +1694,18 @@ vj_status vj_tune(vj_ctx *c, const vj_table *build, const vj_table *probe, const
             best_c = x;
         }
     }
-    if (!e.failed() && !opts->tile && best_c.P > 0) {  // the small tile, for the fastest plan
-        Cand x = best_c;
-        x.tile = 1;
-        double t = 0;
-        e = time_plan(x, &t);
-        if (!e.failed() && t < best_t) {
-            best_t = t;
-            best_c = x;
+    if (!e.failed() && !opts->tile && best_c.P > 0) {  // the other pass tiles, for the fastest plan
+        const Cand top = best_c;
+        for (int tile = 0; tile < kTileShapes && !e.failed(); tile++) {
+            if (tile == top.tile) continue;
+            Cand x = top;
+            x.tile = tile;
+            double t = 0;
+            e = time_plan(x, &t);
+            if (!e.failed() && t < best_t) {
+                best_t = t;
+                best_c = x;
+            }
         }
     }
     (void)hipFree(ob);

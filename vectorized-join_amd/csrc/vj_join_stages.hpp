@@ -125,14 +125,17 @@ struct PassTile {
 
 // Waves per SIMD the pass kernel is compiled for: 8 (64 VGPRs, four blocks per
 // CU) for single 8-byte keys with FNV / XOR hashing; wider keys and the
-// generic hash path get 6 (80 VGPRs, three blocks) to limit spills.
-template <int HP, int KB> constexpr int pass_waves_per_simd() { return KB <= 8 && HP != HP_GENERIC ? 8 : 6; }
+// generic hash path get 6 (80 VGPRs, three blocks) to limit spills; the large
+// tile (more rows per lane) 6, or 5 beyond 8-byte keys.
+template <int HP, int KB, int TV> constexpr int pass_waves_per_simd() {
+    return TV == 2 ? (KB <= 8 ? 6 : 5) : (KB <= 8 && HP != HP_GENERIC ? 8 : 6);
+}
 
 // Pass 1: one tile of a job. Pass 2: one (pass-1 digit, segment) chunk, in
 // tile-sized steps; a running count per digit carries the destination.
 // TV: tile shape (pass_items).
 template <int HP, int KW, int NC, int TV>
-__global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC>())) void k_pass(PipeArgs a, int pass) {
+__global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC, TV>())) void k_pass(PipeArgs a, int pass) {
     using KT = typename KeyT<KW>::T;
     constexpr int KB = KW * NC;
     constexpr int ITEMS = pass_items(KB, TV);
@@ -861,8 +864,10 @@ size_t probe_lds_bytes(int kw, int nc, int table) {
 template <int HP> hipError_t StageLaunch<HP>::hist(const PipeArgs &a, const LaunchShape &sh, hipStream_t s) {
     const size_t lds = (size_t)a.max_hist_bins * 4;
     return dispatch_keys(sh, [&]<int KW, int NC>() {
-        if (sh.tile)
+        if (sh.tile == 1)
             hipLaunchKernelGGL((k_hist<HP, KW, NC, 1>), dim3(a.total_hist_blocks), dim3(kHistThreads), lds, s, a);
+        else if (sh.tile == 2)
+            hipLaunchKernelGGL((k_hist<HP, KW, NC, 2>), dim3(a.total_hist_blocks), dim3(kHistThreads), lds, s, a);
         else
             hipLaunchKernelGGL((k_hist<HP, KW, NC, 0>), dim3(a.total_hist_blocks), dim3(kHistThreads), lds, s, a);
         return hipGetLastError();
@@ -875,8 +880,10 @@ template <int HP> hipError_t StageLaunch<HP>::pass(const PipeArgs &a, const Laun
     const uint32_t grid = pass == 1 ? (total + kXcds - 1) / kXcds * kXcds : (groups + kXcds - 1) / kXcds * kXcds * kXcdGroup;
     const size_t lds = pass_lds_bytes(sh.kw, sh.nc, sh.tile);
     return dispatch_keys(sh, [&]<int KW, int NC>() {
-        if (sh.tile)
+        if (sh.tile == 1)
             hipLaunchKernelGGL((k_pass<HP, KW, NC, 1>), dim3(grid), dim3(kPassThreads), lds, s, a, pass);
+        else if (sh.tile == 2)
+            hipLaunchKernelGGL((k_pass<HP, KW, NC, 2>), dim3(grid), dim3(kPassThreads), lds, s, a, pass);
         else
             hipLaunchKernelGGL((k_pass<HP, KW, NC, 0>), dim3(grid), dim3(kPassThreads), lds, s, a, pass);
         return hipGetLastError();

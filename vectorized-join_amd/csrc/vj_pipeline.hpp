@@ -62,14 +62,18 @@ constexpr uint32_t kXcdGroup = 32;  // consecutive pass-2 chunks kept on one XCD
 // Kernel shapes, chosen per pipeline by the planner or the autotuner.
 //  tile  0: standard pass tile, 512 threads x pass_items rows (6 / 4 / 2 rows by
 //           key bytes); 1: small tile, 512 threads x 2 rows (more blocks for
-//           small tables).
+//           small tables); 2: large tile, 512 threads x 8 / 6 / 4 rows (longer
+//           runs per digit; the region layout's tile).
 //  table 0: standard build+probe block, 512 threads (up to four blocks per
 //           CU) around an LDS table of build_cap rows; 1: wide block, 1024
 //           threads (one block per CU) around a table of wide_cap rows, so a
 //           partition four to eight times larger fits one build chunk.
 __host__ __device__ constexpr int pass_items(int key_bytes, int tile = 0) {
-    return tile ? 2 : (key_bytes <= 8 ? 6 : (key_bytes <= 16 ? 4 : 2));
+    return tile == 1 ? 2
+                     : (tile == 2 ? (key_bytes <= 8 ? 8 : (key_bytes <= 16 ? 6 : 4))
+                                  : (key_bytes <= 8 ? 6 : (key_bytes <= 16 ? 4 : 2)));
 }
+constexpr int kTileShapes = 3;
 __host__ __device__ constexpr int pass_tile(int key_bytes, int tile = 0) { return kPassThreads * pass_items(key_bytes, tile); }
 __host__ __device__ constexpr int build_cap(int key_bytes) { return key_bytes <= 8 ? 2048 : 1024; }
 // Build rows' ids are kept in LDS next to the table for keys wider than 8

@@ -51,7 +51,7 @@ class JoinOpts(ctypes.Structure):
 
 
 # Kernel shapes (vj_join_opts.tile / .table; 0 = planner / autotuned)
-TILE_STANDARD, TILE_SMALL = 1, 2
+TILE_STANDARD, TILE_SMALL, TILE_LARGE = 1, 2, 3
 TABLE_STANDARD, TABLE_WIDE = 1, 2
 # Partition layouts (vj_join_opts.layout; 0 = planner / autotuned)
 LAYOUT_EXACT, LAYOUT_REGIONS = 1, 2

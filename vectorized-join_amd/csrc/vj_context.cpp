@@ -270,8 +270,8 @@ uint64_t region_rows(uint64_t mean) {
     return (r + 63) & ~63ull;
 }
 
-// Region sizes of a two-pass job of n rows (false: its row indexes would not
-// fit 32 bits).
+// Region sizes of a two-pass job of n rows (false: its buffers would pass
+// VJ_MAX_ROWS rows, the bound every 32-bit row index of the kernels keeps).
 struct RegionShape {
     uint32_t nseg, seg_tiles, region1, region2, region_tiles;
     uint64_t mid_rows, fin_rows;
@@ -292,7 +292,7 @@ bool region_shape(uint64_t n, int b1, int b2, uint32_t T, RegionShape *r) {
     r->region1 = (uint32_t)r1;
     r->region2 = (uint32_t)r2;
     r->region_tiles = (uint32_t)((r1 + T - 1) / T);
-    return r->mid_rows < (1ull << 32) && r->fin_rows < (1ull << 32);
+    return r->mid_rows <= VJ_MAX_ROWS && r->fin_rows <= VJ_MAX_ROWS;
 }
 
 void add_job(Plan &pl, const vj_table *t, int P, int b1, int b2, uint32_t shift0, bool fin_external = false) {

@@ -460,6 +460,20 @@ struct Emitter {
 // next iteration are loaded before the current one is probed, so one HBM round
 // trip overlaps each iteration's probing (where registers allow, see PF).
 
+// A key's place in the LDS table: its 16-bit tag (never 0) << 16 | its bucket,
+// from the hash bits above the partition digits (the digits are the same for
+// every row of a partition). FNV takes the tag from the top 16 hash bits.
+// custom_xor_hw's 64-bit hash of 4-byte keys leaves the upper 32 bits zero
+// (every key would get the same tag), so it and the generic path take the tag
+// from a multiplicative hash of the 32 bits above the digits, whose top bits
+// depend on all of them.
+template <int HP>
+__device__ __forceinline__ uint32_t table_place(uint64_t h, uint32_t shift, uint32_t tag_shift, uint32_t bmask) {
+    const uint32_t y = (uint32_t)(h >> shift);
+    const uint32_t tag = HP == HP_FNV ? (uint32_t)(h >> tag_shift) & 0xFFFFu : (y * 0x9E3779B1u) >> 16;
+    return ((tag | 1u) << 16) | (y & bmask);
+}
+
 template <int N> __device__ __forceinline__ uint32_t res_get(const uint32_t (&r)[N], int k) {
     return (r[k >> 1] >> ((k & 1) * 16)) & 0xFFFFu;
 }
@@ -533,7 +547,7 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const uint64_t h = hash_key<HP, KW, NC>(sk[u], hc);
-            st[u] = ((((uint32_t)(h >> tag_shift) & 0xFFFFu) | 1u) << 16) | ((uint32_t)(h >> shift) & bmask);
+            st[u] = table_place<HP>(h, shift, tag_shift, bmask);
         }
         while (__ballot(act != 0)) {
             uint4 E[U];
@@ -784,9 +798,10 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
 #pragma unroll
             for (int c = 0; c < NC; c++) k.c[c] = rk[c * CAP + i];
             const uint64_t h = hash_key<HP, KW, NC>(k, hc);
-            const uint32_t tag = ((uint32_t)(h >> tag_shift) & 0xFFFFu) | 1u;  // never 0: an entry is never 0
+            const uint32_t place = table_place<HP>(h, shift, tag_shift, bmask);  // tag never 0: an entry is never 0
+            const uint32_t tag = place >> 16;
             const uint32_t e = (tag << 16) | (i + 1);
-            uint32_t b = (uint32_t)(h >> shift) & bmask;
+            uint32_t b = place & 0xFFFFu;
             // Scan slots in order (the bucket, then the next buckets). An empty
             // slot takes the row as a new entry; the entry of an equal key takes
             // it as its new chain head (nxt[i] = old head). nxt[i] is written

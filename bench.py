@@ -672,8 +672,9 @@ def bigbuild_leg(ctx, steps=5, log_r=26, log_s=27):
     past two passes' 14; the planner's wide build+probe block (1024 threads,
     8192-row tables) keeps one build chunk per partition at 14 bits. Timed
     against the same join forced onto the standard block (several build chunks
-    per partition, so every probe chunk is read once per build chunk). Checked
-    as the FK join: every probe row once, with an equal key."""
+    per partition, so every probe chunk is read once per build chunk), and the
+    wide block in the region layout (no histogram read). Checked as the FK
+    join: every probe row once, with an equal key."""
     import numpy as np
     import torch
 
@@ -687,8 +688,9 @@ def bigbuild_leg(ctx, steps=5, log_r=26, log_s=27):
     out_s = torch.empty(ns, dtype=torch.int32, device="cuda")
     res = {"workload": "build 2^%d unique int64 keys, FK probe 2^%d rows, FNV-1a-64, 2 radix passes" % (log_r, log_s),
            "build_rows": nr, "probe_rows": ns}
-    for name, table in (("wide_table", 0), ("standard_table", 1)):
-        opts = vjoin.join_opts("fnv", 64, 64, table=table)
+    for name, table, layout in (("wide_table", 0, 0), ("wide_table_regions", 0, vjoin.LAYOUT_REGIONS),
+                                ("standard_table", 1, 0)):
+        opts = vjoin.join_opts("fnv", 64, 64, table=table, layout=layout)
         b, p = vjoin.table(dr), vjoin.table(ds)
         ms, stages = [], {k: [] for k in ("ms_histogram", "ms_pass1", "ms_pass2", "ms_build_probe")}
         for i in range(steps + 1):
@@ -709,6 +711,8 @@ def bigbuild_leg(ctx, steps=5, log_r=26, log_s=27):
         res[name] = {"ms": t, "stage_ms": {k[3:]: float(np.median(v)) for k, v in stages.items()},
                      "passes": sm["passes"], "bits": [sm["bits1"], sm["bits2"]], "probe_threads": sm["probe_threads"],
                      "table_rows": sm["table_rows"], "tasks": sm["tasks"], "probe_tuples_per_s": ns / (t * 1e-3),
+                     "layout": "regions" if sm["layout"] == vjoin.LAYOUT_REGIONS else "exact",
+                     "tile_rows": sm["tile_rows"],
                      "algorithmic_bytes": B, "frac": B / (t * 1e-3) / HBM_PEAK, "parity": bool(ok)}
     return res
 

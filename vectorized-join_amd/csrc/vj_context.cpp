@@ -830,20 +830,21 @@ void drop_graph(vj_ctx *c) {
 
 // Captures upload -> all stages -> count download on the context's capture
 // stream (capture needs a stream other than the null stream) and instantiates
-// it. Events are recorded around the graph launch, not inside the graph
-// (event nodes inside a replayed graph gave stage times that did not add up).
+// it. The upload and the download are kernels that read / write the pinned
+// staging buffer (no copy nodes: a replay stays on the compute queue). Events
+// are recorded around the graph launch, not inside the graph (event nodes
+// inside a replayed graph gave stage times that did not add up).
 Err capture_pipeline(vj_ctx *c, const Plan &pl, const PipeArgs &a) {
     drop_graph(c);
     if (!c->capture_stream) VJ_HIP_TRY(hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking));
     hipStream_t cs = c->capture_stream;
     char *h = static_cast<char *>(c->pinned);
     VJ_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
-    hipError_t he = hipMemcpyAsync(c->meta.p, h, pl.upload_bytes, hipMemcpyHostToDevice, cs);
+    hipError_t he = launch_stage_copy(c->meta.p, h, pl.upload_bytes, cs);
     Err e = he == hipSuccess ? launch_stages(c, pl, a, cs, true, false)
                              : Err::make(VJ_ERR_HIP, std::string("capture upload: ") + hipGetErrorString(he));
     if (!e.failed()) {
-        he = hipMemcpyAsync(h + pl.off_count, static_cast<char *>(c->meta.p) + pl.off_count, pl.down_bytes(),
-                            hipMemcpyDeviceToHost, cs);
+        he = launch_stage_copy(h + pl.off_count, static_cast<char *>(c->meta.p) + pl.off_count, pl.down_bytes(), cs);
         if (he != hipSuccess) e = Err::make(VJ_ERR_HIP, std::string("capture download: ") + hipGetErrorString(he));
     }
     hipGraph_t g = nullptr;

@@ -377,6 +377,16 @@ __global__ __launch_bounds__(1024) void k_region_reset(PipeArgs a) {
     for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) m2[i] = 0;
 }
 
+// Copies 8-byte words between device memory and the context's pinned staging
+// buffer (device-accessible host memory). A captured pipeline uploads its
+// descriptors and downloads its counts with it instead of copy nodes, so a
+// replay never leaves the compute queue.
+__global__ __launch_bounds__(256) void k_stage_copy(unsigned long long *dst, const unsigned long long *src,
+                                                    uint64_t words) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // Resets the queue and the output counters to run build+probe again (after
 // an output buffer was found too small).
 __global__ void k_reset_probe(PipeArgs a) {
@@ -456,6 +466,16 @@ hipError_t launch_tasks(const PipeArgs &a, hipStream_t s) {
         e = hipGetLastError();
     }
     return e;
+}
+
+hipError_t launch_stage_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (bytes % 8) return hipErrorInvalidValue;
+    const uint64_t words = bytes / 8;
+    if (!words) return hipSuccess;
+    const uint64_t blocks = (words + 255) / 256 < 1024 ? (words + 255) / 256 : 1024;
+    hipLaunchKernelGGL(k_stage_copy, dim3((unsigned)blocks), dim3(256), 0, s, static_cast<unsigned long long *>(dst),
+                       static_cast<const unsigned long long *>(src), words);
+    return hipGetLastError();
 }
 
 hipError_t launch_reset_probe(const PipeArgs &a, hipStream_t s) {

@@ -224,6 +224,7 @@ hipError_t launch_scan(const PipeArgs &a, hipStream_t s);
 hipError_t launch_region_reset(const PipeArgs &a, hipStream_t s);
 hipError_t launch_tasks(const PipeArgs &a, hipStream_t s);
 hipError_t launch_reset_probe(const PipeArgs &a, hipStream_t s);
+hipError_t launch_stage_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_pass(const PipeArgs &a, const LaunchShape &sh, int pass, hipStream_t s);
 hipError_t launch_probe(const PipeArgs &a, const LaunchShape &sh, int blocks, hipStream_t s);
 int probe_blocks_per_cu(const LaunchShape &sh);

@@ -324,26 +324,28 @@ def batched_leg(ctx, n_joins, reps, seed=None):
             counts.append(cnt)
         return counts
 
-    # warm-up (allocates the arenas, captures the graph), then timed runs
+    # warm-up (allocates the arenas, captures the graph), then each mode timed
+    # as a block of back-to-back runs after one untimed run of that mode (the
+    # first batch after 256 plain joins runs ~0.25 ms slower in either mode)
     c_plain = plain()
     sum_plain = _pair_checksums(outs_r, outs_s, c_plain)
     c_batch = batch(True)
     sum_batch = _pair_checksums(outs_r, outs_s, c_batch)
-    t_plain, t_batch, t_eager = [], [], []
-    for _ in range(reps):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        plain()
-        torch.cuda.synchronize()
-        t_plain.append(time.perf_counter() - t0)
-        t0 = time.perf_counter()
-        batch(True)
-        torch.cuda.synchronize()
-        t_batch.append(time.perf_counter() - t0)
-        t0 = time.perf_counter()
-        batch(False)
-        torch.cuda.synchronize()
-        t_eager.append(time.perf_counter() - t0)
+
+    def timed_block(f):
+        f()
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            f()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return ts
+
+    t_plain = timed_block(plain)
+    t_batch = timed_block(lambda: batch(True))
+    t_eager = timed_block(lambda: batch(False))
     c_last = batch(True)
     same = c_plain == c_batch == c_last and sum_plain == sum_batch == _pair_checksums(outs_r, outs_s, c_last)
     tp, tb, te = float(np.median(t_plain)), float(np.median(t_batch)), float(np.median(t_eager))

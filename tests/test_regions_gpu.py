@@ -186,3 +186,52 @@ def test_layout_out_of_range_is_rejected(ctx):
     o = vjoin.join_opts(layout=3)
     rc, _ = ctx.join_into(vjoin.table([d]), vjoin.table([d]), o, out, out)
     assert rc == vjoin.STATUS_CODE["VJ_ERR_INVALID_ARGUMENT"]
+
+
+@pytest.mark.parametrize("kw,nc,signed,one_based", [(4, 3, True, True), (8, 3, False, True), (8, 1, True, False)])
+def test_region_layout_with_row_ids_one_based_and_signed_keys(ctx, kw, nc, signed, one_based):
+    """Explicit row ids, 1-based output, signed keys and three key columns (keys up to 24 bytes, wide table)."""
+    import torch
+    rng = np.random.default_rng(kw * 10 + nc)
+    r, s = _fk(rng, 70000, 400000, kw, nc)
+    rr = rng.permutation(1 << 24)[:len(r)].astype(np.uint32)
+    sr = rng.permutation(1 << 24)[:len(s)].astype(np.uint32)
+    ob = 1 if one_based else 0
+    want = np_join(r.astype(np.uint64), s.astype(np.uint64), rr + ob, sr + ob)
+    dr = _dev([r[:, c] for c in range(nc)], kw)
+    ds = _dev([s[:, c] for c in range(nc)], kw)
+    drr = torch.from_numpy(rr.view(np.int32)).cuda()
+    dsr = torch.from_numpy(sr.view(np.int32)).cuda()
+    opts = vjoin.join_opts(passes=2, bits1=5, bits2=4, one_based=one_based, table=vjoin.TABLE_WIDE,
+                           layout=vjoin.LAYOUT_REGIONS)
+    a, b = ctx.join(vjoin.table(dr, drr, signed), vjoin.table(ds, dsr, signed), opts)
+    sm = ctx.summary()
+    assert (sm["layout"], sm["layout_fallback"]) == (vjoin.LAYOUT_REGIONS, 0), sm
+    _check(_pairs(a, b), want, (kw, nc, signed, one_based))
+
+
+def test_paths_that_keep_the_exact_layout(ctx):
+    """The region layout asked for on the no-allocation entry point and on a
+    resident build side: both run the exact layout and give the same pairs."""
+    import torch
+    rng = np.random.default_rng(33)
+    r, s = _fk(rng, 1 << 15, 1 << 18, 8, 1)
+    want = np_join(r, s)
+    dr, ds = _dev([r[:, 0]], 8), _dev([s[:, 0]], 8)
+    opts = vjoin.join_opts(passes=2, bits1=4, bits2=4, layout=vjoin.LAYOUT_REGIONS)
+    m = len(want[0])
+    o_r = torch.empty(m, dtype=torch.int32, device="cuda")
+    o_s = torch.empty(m, dtype=torch.int32, device="cuda")
+    rc, cnt = ctx.join_into_noalloc(vjoin.table(dr), vjoin.table(ds), opts, o_r, o_s)
+    assert rc == 0 and cnt == m, ctx.last_error()
+    assert ctx.summary()["layout"] == vjoin.LAYOUT_EXACT
+    _check(_pairs(o_r, o_s), want, "no-allocation entry point")
+    built = ctx.build(vjoin.table(dr), opts)
+    try:
+        o_r.fill_(-1)
+        rc, cnt = ctx.probe_into(built, vjoin.table(ds), opts, o_r, o_s)
+        assert rc == 0 and cnt == m, ctx.last_error()
+        assert ctx.summary()["layout"] == vjoin.LAYOUT_EXACT
+        _check(_pairs(o_r, o_s), want, "resident build side")
+    finally:
+        built.free()

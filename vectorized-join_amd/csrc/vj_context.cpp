@@ -1000,7 +1000,7 @@ Err run_joins(vj_ctx *c, std::vector<JoinSpec> &specs, const vj_join_opts *o, hi
         vj_join_opts ox = *o;
         ox.layout = VJ_LAYOUT_EXACT;
         VJ_TRY(run_joins(c, specs, &ox, s, counts, grow, extra_bytes, retries_out, gmode));
-        c->summary.layout_fallback = 1;
+        if (specs.size() == 1) c->summary.layout_fallback = 1;  // the summary describes single joins
         return Err::ok();
     }
     counts.assign(hc, hc + pl.joins.size());

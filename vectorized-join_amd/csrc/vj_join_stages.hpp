@@ -208,10 +208,10 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC, TV>
                         ti.trash = (job.nseg << job.b1) * job.region1;
                     }
                 } else if (job.region1) {
-                    // chunk = tile k of pass-1 region (d1, g); digit d2's region is (d1 << b2) | d2 of fin
+                    // chunk = tile kt of pass-1 region (d1, g); digit d2's region is (d1 << b2) | d2 of fin
                     const uint32_t per = job.nseg * job.region_tiles;
                     const uint32_t d1 = local / per, r = local - d1 * per;
-                    const uint32_t g = r / job.region_tiles, k = r - g * job.region_tiles;
+                    const uint32_t g = r / job.region_tiles, kt = r - g * job.region_tiles;
                     const uint32_t c1 = a.counts[job.m1_off + d1 * job.nseg + g];
                     const uint32_t filled = c1 < job.region1 ? c1 : job.region1;
                     const uint32_t start = (d1 * job.nseg + g) * job.region1;
@@ -220,9 +220,9 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC, TV>
                     ti.src.n = job.n;
                     for (int c = 0; c < 4; c++) ti.dst_cols[c] = job.fin_cols[c];
                     ti.dst_ids = job.fin_ids;
-                    ti.lo = start + k * T;
-                    ti.hi = start + ((k + 1) * T < filled ? (k + 1) * T : filled);
-                    ti.valid = k * T < filled;
+                    ti.lo = start + kt * T;
+                    ti.hi = start + ((kt + 1) * T < filled ? (kt + 1) * T : filled);
+                    ti.valid = kt * T < filled;
                     ti.bins = 1u << job.b2;
                     ti.shift = job.shift0 + job.b1;
                     ti.region = job.region2;

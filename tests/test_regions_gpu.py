@@ -235,3 +235,68 @@ def test_paths_that_keep_the_exact_layout(ctx):
         _check(_pairs(o_r, o_s), want, "resident build side")
     finally:
         built.free()
+
+
+N_RANDOM = 48
+
+
+@pytest.mark.parametrize("seed", range(N_RANDOM))
+def test_random_region_configurations(ctx, seed):
+    """Seeded random two-pass plans in the region layout: key width and columns,
+    hash kind, radix bits, pass tile, table shape, duplicate density (dense
+    duplicates overflow and fall back), row ids, 1-based output, signed keys
+    and the entry point; every case against the numpy join."""
+    import torch
+    rng = np.random.default_rng(7000 + seed)
+    kw = int(rng.choice([4, 8]))
+    nc = int(rng.integers(1, 5))
+    kind = str(rng.choice(["fnv", "custom_xor_hw", "custom_add", "custom_mult"]))
+    b1 = int(rng.integers(3, 9))
+    b2 = int(rng.integers(1, min(8, 14 - b1) + 1))
+    nr, ns = int(2 ** rng.uniform(0, 17)), int(2 ** rng.uniform(0, 19))
+    dens = float(rng.choice([0.05, 0.5, 1.0, 20.0]))
+    tile, table = int(rng.integers(0, 4)), int(rng.integers(0, 3))
+    rowids, one_based, signed = bool(rng.random() < 0.3), bool(rng.random() < 0.3), bool(rng.random() < 0.2)
+    entry = str(rng.choice(["join", "into", "batch"]))
+    cfg = dict(seed=seed, kw=kw, nc=nc, kind=kind, b1=b1, b2=b2, nr=nr, ns=ns, dens=dens, tile=tile, table=table,
+               rowids=rowids, one_based=one_based, signed=signed, entry=entry)
+    print("case", cfg, flush=True)
+    dt = np.uint32 if kw == 4 else np.uint64
+    hi = np.iinfo(dt).max
+    nd = max(1, int(nr * dens))
+    distinct = rng.integers(0, hi, size=(nd, nc), dtype=np.uint64, endpoint=True).astype(dt)
+    r = distinct[rng.integers(0, nd, size=nr)]
+    s = distinct[rng.integers(0, nd, size=ns)]
+    miss = rng.random(ns) < 0.3
+    s[miss, 0] = rng.integers(0, hi, size=int(miss.sum()), dtype=np.uint64, endpoint=True).astype(dt)
+    rr = rng.permutation(1 << 24)[:nr].astype(np.uint32) if rowids else None
+    sr = rng.permutation(1 << 24)[:ns].astype(np.uint32) if rowids else None
+    ob = 1 if one_based else 0
+    want = np_join(r.astype(np.uint64), s.astype(np.uint64), None if rr is None else rr + ob,
+                   None if sr is None else sr + ob)
+    if rr is None and ob:
+        want = (want[0] + np.uint32(1), want[1] + np.uint32(1))
+    m = len(want[0])
+    dr = _dev([r[:, c] for c in range(nc)], kw)
+    ds = _dev([s[:, c] for c in range(nc)], kw)
+    drr = None if rr is None else torch.from_numpy(rr.view(np.int32)).cuda()
+    dsr = None if sr is None else torch.from_numpy(sr.view(np.int32)).cuda()
+    bt, pt = vjoin.table(dr, drr, signed), vjoin.table(ds, dsr, signed)
+    opts = vjoin.join_opts(kind, 64, 64, passes=2, bits1=b1, bits2=b2, one_based=one_based, tile=tile, table=table,
+                           layout=vjoin.LAYOUT_REGIONS)
+    if entry == "join":
+        got = _pairs(*ctx.join(bt, pt, opts))
+        assert ctx.summary()["layout"] in (vjoin.LAYOUT_REGIONS, vjoin.LAYOUT_EXACT)
+    elif entry == "into":
+        o_r = torch.empty(max(m, 1), dtype=torch.int32, device="cuda")
+        o_s = torch.empty(max(m, 1), dtype=torch.int32, device="cuda")
+        rc, cnt = ctx.join_into(bt, pt, opts, o_r, o_s, capacity=m)
+        assert rc == 0 and cnt == m, (rc, cnt, m, ctx.last_error())
+        got = _pairs(o_r[:cnt], o_s[:cnt])
+    else:  # the second of two copies in one batch
+        outs = [(torch.empty(max(m, 1), dtype=torch.int32, device="cuda"),
+                 torch.empty(max(m, 1), dtype=torch.int32, device="cuda")) for _ in range(2)]
+        rc, counts = ctx.join_batch([bt, bt], [pt, pt], opts, [o[0] for o in outs], [o[1] for o in outs], [m, m])
+        assert rc == 0 and list(counts) == [m, m], (rc, counts, ctx.last_error())
+        got = _pairs(outs[1][0][:m], outs[1][1][:m])
+    _check(got, want, cfg)

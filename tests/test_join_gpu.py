@@ -302,8 +302,10 @@ def test_vj_join_grows_output_beyond_hint(ctx):
     _assert_same(_pairs(*got), want)
 
 
-@pytest.mark.parametrize("bits,shift", [(8, 0), (5, 3), (1, 17)])
+@pytest.mark.parametrize("bits,shift", [(8, 0), (5, 3), (1, 17), (6, 28), (4, 40)])
 def test_partition_matches_oracle(ctx, bits, shift):
+    """Digits below bit 32 come from the hash's low half (32-bit arithmetic for
+    FNV-1a-64); digits reaching past it from the full 64-bit hash."""
     import torch
     rng = np.random.default_rng(bits)
     k = rng.integers(0, 2**63, size=100000, dtype=np.uint64)

@@ -346,6 +346,33 @@ template <typename HT, int NC> __device__ __forceinline__ HT xor_hw_zext_cols(co
     return h;
 }
 
+// FNV-1a with the given constants in 32-bit arithmetic. With the truncated
+// 64-bit offset basis and prime this is the low half of FNV-1a-64 (a product's
+// low 32 bits depend only on its factors' low 32 bits, and every word is 32
+// bits wide), at one 32-bit multiply per word instead of a 64-bit one.
+template <int NW>
+__device__ __forceinline__ uint32_t fnv_words_lo(const uint32_t (&w)[NW], int n, uint32_t off, uint32_t prime) {
+    uint32_t x = off;
+#pragma unroll
+    for (int i = 0; i < NW; i++) {
+        if (i < n) {
+            x ^= w[i];
+            x *= prime;
+        }
+    }
+    return x;
+}
+template <int NC>
+__device__ __forceinline__ uint32_t fnv_zext_cols_lo(const uint32_t (&k)[NC], uint32_t off, uint32_t prime2) {
+    uint32_t x = off;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        x ^= k[c];
+        x *= prime2;
+    }
+    return x;
+}
+
 // Hash of a key as a uint64 (32-bit hashes zero-extended).
 template <int HP, int KW, int NC>
 __device__ __forceinline__ uint64_t hash_key(const Key<KW, NC> &k, const HashCfg &h) {
@@ -374,6 +401,29 @@ __device__ __forceinline__ uint64_t hash_key(const Key<KW, NC> &k, const HashCfg
     } else {
         return h.bits == 32 ? (uint64_t)generic_words<uint32_t, 2 * NC>(w, n, h.kind, h.chunk_bits)
                             : generic_words<uint64_t, 2 * NC>(w, n, h.kind, h.chunk_bits);
+    }
+}
+
+// The low 32 bits of hash_key (for digits and table places below bit 32):
+// FNV-1a in 32-bit arithmetic, the other kinds by truncation.
+template <int HP, int KW, int NC>
+__device__ __forceinline__ uint32_t hash_key_lo(const Key<KW, NC> &k, const HashCfg &h) {
+    if constexpr (HP == HP_FNV) {
+        const bool b32 = h.bits == 32;
+        const uint32_t off = b32 ? Fnv<uint32_t>::offset : (uint32_t)Fnv<uint64_t>::offset;
+        const uint32_t prime = b32 ? Fnv<uint32_t>::prime : (uint32_t)Fnv<uint64_t>::prime;
+        if (KW == 4 && !h.key_signed) {
+            uint32_t c[NC];
+#pragma unroll
+            for (int i = 0; i < NC; i++) c[i] = (uint32_t)k.c[i];
+            if (!h.native) return fnv_zext_cols_lo<NC>(c, off, b32 ? Fnv<uint32_t>::prime2 : (uint32_t)Fnv<uint64_t>::prime2);
+            return fnv_words_lo<NC>(c, NC, off, prime);
+        }
+        uint32_t w[2 * NC];
+        const int n = key_words<KW, NC>(k, h, w);
+        return fnv_words_lo<2 * NC>(w, n, off, prime);
+    } else {
+        return (uint32_t)hash_key<HP, KW, NC>(k, h);
     }
 }
 

@@ -51,6 +51,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(PipeArgs a) {
     __syncthreads();
     const uint32_t m1 = n1 - 1, m2 = (1u << job.b2) - 1, b2 = job.b2;
     const uint32_t s1 = job.shift0, s2 = job.shift0 + job.b1;
+    const bool lo32 = s2 + b2 <= 32;  // every digit below bit 32 (joins): the low half of the hash suffices
     const HashCfg hc = a.hash;
     // histogram row g: the hrow_tiles tiles of segment g / hsplit starting at
     // its (g % hsplit)-th row (the last rows of a segment may be short or empty)
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(PipeArgs a) {
             for (int u = 0; u < U; u++) {
                 const uint32_t i = row_lo + u * kHistThreads;
                 if (i < n) {
-                    const uint64_t h = hash_key<HP, KW, NC>(key[u], hc);
+                    const uint64_t h = lo32 ? (uint64_t)hash_key_lo<HP, KW, NC>(key[u], hc) : hash_key<HP, KW, NC>(key[u], hc);
                     const uint32_t d1 = (uint32_t)(h >> s1) & m1;
                     atomicAdd(&thk[d1], 1u);
                     if (two) atomicAdd(&h2[(d1 << b2) | ((uint32_t)(h >> s2) & m2)], 1u);
@@ -268,6 +269,7 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC, TV>
     }
 
     const uint32_t bins = ti.bins, shift = ti.shift, mask = bins - 1;
+    const bool lo32 = shift + (uint32_t)__builtin_ctz(bins) <= 32;  // the digit lies in the hash's low half
     const GKeys<KW, NC> skeys_g = uniform_keys<KW, NC>(ti.src);  // source columns, in scalar registers
     const gptr<const uint32_t> sids_g = uniform_global(ti.src.rowids);
     gptr<KT> dcols[NC];  // destination columns
@@ -306,7 +308,10 @@ __global__ __launch_bounds__(kPassThreads, (pass_waves_per_simd<HP, KW * NC, TV>
             const uint32_t i = wbase + it * 64;
             const bool valid = i < hi;
             uint32_t d = 0;
-            if (valid) d = (uint32_t)(hash_key<HP, KW, NC>(keys[it], hc) >> shift) & mask;
+            if (valid) {
+                const uint64_t h = lo32 ? (uint64_t)hash_key_lo<HP, KW, NC>(keys[it], hc) : hash_key<HP, KW, NC>(keys[it], hc);
+                d = (uint32_t)(h >> shift) & mask;
+            }
             const uint32_t r = valid ? atomicAdd(&wcnt[w * 256 + d], 1u) : 0u;
             pk[it] = valid ? (d | (r << 8)) : ~0u;
         }
@@ -461,17 +466,15 @@ struct Emitter {
 // trip overlaps each iteration's probing (where registers allow, see PF).
 
 // A key's place in the LDS table: its 16-bit tag (never 0) << 16 | its bucket,
-// from the hash bits above the partition digits (the digits are the same for
-// every row of a partition). FNV takes the tag from the top 16 hash bits.
-// custom_xor_hw's 64-bit hash of 4-byte keys leaves the upper 32 bits zero
-// (every key would get the same tag), so it and the generic path take the tag
-// from a multiplicative hash of the 32 bits above the digits, whose top bits
-// depend on all of them.
-template <int HP>
-__device__ __forceinline__ uint32_t table_place(uint64_t h, uint32_t shift, uint32_t tag_shift, uint32_t bmask) {
-    const uint32_t y = (uint32_t)(h >> shift);
-    const uint32_t tag = HP == HP_FNV ? (uint32_t)(h >> tag_shift) & 0xFFFFu : (y * 0x9E3779B1u) >> 16;
-    return ((tag | 1u) << 16) | (y & bmask);
+// from the low 32 hash bits above the partition digits (the digits are the
+// same for every row of a partition; digits and buckets end below bit 27).
+// The tag is the top of a multiplicative hash of those bits, which depends on
+// all of them (the top 16 bits of a 64-bit hash would not do: custom_xor_hw's
+// 64-bit hash of 4-byte keys has a zero upper half, so every key would get
+// the same tag).
+__device__ __forceinline__ uint32_t table_place(uint32_t lo, uint32_t shift, uint32_t bmask) {
+    const uint32_t y = lo >> shift;
+    return ((((y * 0x9E3779B1u) >> 16) | 1u) << 16) | (y & bmask);
 }
 
 template <int N> __device__ __forceinline__ uint32_t res_get(const uint32_t (&r)[N], int k) {
@@ -515,7 +518,7 @@ template <int N> __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 }
 
 template <int HP, int KW, int NC, int PV>
-__device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg &hc, uint32_t tag_shift,
+__device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg &hc,
                                                const typename KeyT<KW>::T *rk, const uint32_t *slots,
                                                const uint16_t *nxt, uint32_t (&res)[ProbeShape<PV>::RES]) {
     constexpr int CAP = probe_cap(KW * NC, PV);
@@ -545,10 +548,7 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
         }
         if (PF && it + 1 < ITERS) walk_load<U, TH, KW, NC>(wa, it + 1, nk);
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const uint64_t h = hash_key<HP, KW, NC>(sk[u], hc);
-            st[u] = table_place<HP>(h, shift, tag_shift, bmask);
-        }
+        for (int u = 0; u < U; u++) st[u] = table_place(hash_key_lo<HP, KW, NC>(sk[u], hc), shift, bmask);
         while (__ballot(act != 0)) {
             uint4 E[U];
 #pragma unroll
@@ -693,7 +693,6 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
     em.pos = 0;
     const HashCfg hc = a.hash;
     const unsigned long long total = a.ctl->task_total;
-    const uint32_t tag_shift = hc.bits - 16;
 
     // Lane 0 of wave 0 holds the ticket of the block's next task, fetched
     // while the current one runs.
@@ -797,8 +796,7 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
             Key<KW, NC> k;
 #pragma unroll
             for (int c = 0; c < NC; c++) k.c[c] = rk[c * CAP + i];
-            const uint64_t h = hash_key<HP, KW, NC>(k, hc);
-            const uint32_t place = table_place<HP>(h, shift, tag_shift, bmask);  // tag never 0: an entry is never 0
+            const uint32_t place = table_place(hash_key_lo<HP, KW, NC>(k, hc), shift, bmask);  // tag never 0: an entry is never 0
             const uint32_t tag = place >> 16;
             const uint32_t e = (tag << 16) | (i + 1);
             uint32_t b = place & 0xFFFFu;
@@ -834,7 +832,7 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
         // find and count every row's matches (kept in registers), reserve the
         // task's output rows with one atomic, then write them
         const WalkArgs wa = walk_args(t);
-        const uint32_t wc = probe_walk<HP, KW, NC, PV>(wa, hc, tag_shift, rk, slots, nxt, res);
+        const uint32_t wc = probe_walk<HP, KW, NC, PV>(wa, hc, rk, slots, nxt, res);
         if (lane == 0) t.wave_count[w] = wc;
         __syncthreads();
         if (tid == 0) {

@@ -395,7 +395,6 @@ struct ProbeTask {
 
 constexpr int kProbeUnroll = 2;
 constexpr int kBucketSlots = 4;  // LDS table: buckets of 4 slots (one 16-byte load)
-constexpr uint16_t kNoMatch = 0xFFFF;
 
 // One task's parameters for the walks (all wave-uniform).
 struct WalkArgs {
@@ -490,6 +489,7 @@ template <int PV> struct ProbeShape {
     static constexpr int WAVES = TH / 64;
     static constexpr int ITERS = kProbeChunk / (TH * kProbeUnroll);
     static constexpr int RES = ITERS * kProbeUnroll / 2;  // words of packed 16-bit results per lane
+    static_assert(kProbeUnroll % 2 == 0, "an iteration's results fill whole words");
     static_assert(ITERS * TH * kProbeUnroll == kProbeChunk && WAVES <= kMaxProbeWaves, "probe chunk shape");
 };
 
@@ -537,25 +537,30 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
         if (walk_wave_row<U, TH>(wa, it) >= s_end) break;  // wave-uniform
         const uint32_t row = walk_row<U, TH>(wa, it);
         Key<KW, NC> sk[U];
-        uint32_t st[U];   // 16-bit tag << 16 | bucket (buckets < 2^16)
-        uint32_t act = 0;  // bit u: row u of this lane still probing
+        // 16-bit tag << 16 | first bucket (buckets < 2^16); round r reads bucket
+        // first + r. 0 once the row is done (or past the chunk): tag 0 matches
+        // only empty slots, so such a row finds nothing and stays done.
+        uint32_t st[U];
         if (!PF && it > 0) walk_load<U, TH, KW, NC>(wa, it, nk);
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            sk[u] = nk[u];
-            res_set(res, it * U + u, kNoMatch);
-            act |= (row + u * 64 < s_end ? 1u : 0u) << u;
-        }
+        for (int u = 0; u < U; u++) sk[u] = nk[u];
+#pragma unroll
+        for (int u = 0; u < U; u += 2) res[(it * U + u) >> 1] = 0;  // a row is found at most once: its field is or-ed in
         if (PF && it + 1 < ITERS) walk_load<U, TH, KW, NC>(wa, it + 1, nk);
 #pragma unroll
-        for (int u = 0; u < U; u++) st[u] = table_place(hash_key_lo<HP, KW, NC>(sk[u], hc), shift, bmask);
-        while (__ballot(act != 0)) {
+        for (int u = 0; u < U; u++)
+            st[u] = row + u * 64 < s_end ? table_place(hash_key_lo<HP, KW, NC>(sk[u], hc), shift, bmask) : 0u;
+        uint32_t rnd = 0;
+        while (true) {
+            uint32_t any = 0;
+#pragma unroll
+            for (int u = 0; u < U; u++) any |= st[u];
+            if (!__ballot(any != 0)) break;
             uint4 E[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) E[u] = *reinterpret_cast<const uint4 *>(slots + ((st[u] & 0xFFFFu) << 2));
+            for (int u = 0; u < U; u++) E[u] = *reinterpret_cast<const uint4 *>(slots + (((st[u] + rnd) & bmask) << 2));
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const bool live = (act >> u) & 1u;
                 const uint32_t tg = st[u] >> 16;
                 const uint32_t e4[kBucketSlots] = {E[u].x, E[u].y, E[u].z, E[u].w};
                 // the first entry whose tag matches (0: none; tags are never 0, so
@@ -566,7 +571,6 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
 #pragma unroll
                 for (int j = kBucketSlots - 1; j >= 0; j--) ej = (e4[j] >> 16) == tg ? e4[j] : ej;
                 const bool emp = e4[kBucketSlots - 1] == 0;
-                if (!live) ej = 0;
                 uint32_t fo = ej ? (ej & 0xFFFFu) - 1 : 0u;
                 bool found = ej != 0;
 #pragma unroll
@@ -592,8 +596,7 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
                         }
                     }
                 }
-                const uint32_t cur = res_get(res, it * U + u);
-                res_set(res, it * U + u, found ? fo : cur);
+                res[(it * U + u) >> 1] |= (found ? fo + 1 : 0u) << (((it * U + u) & 1) * 16);
                 uint32_t n = found ? 1u : 0u;
                 if (wa.dup) {  // wave-uniform: equal keys of the build chunk are chained behind the entry
                     uint32_t x = found ? nxt[fo] : 0u;
@@ -603,9 +606,9 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
                     }
                 }
                 total += n;
-                act &= ~((live && (found || emp) ? 1u : 0u) << u);
-                st[u] = (st[u] & 0xFFFF0000u) | ((st[u] + 1) & bmask);
+                st[u] = found || emp ? 0u : st[u];
             }
+            rnd++;
         }
     }
     return wave_sum<64>(total);
@@ -636,9 +639,9 @@ __device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint16_t *
             for (int u = 0; u < U; u++) {
                 const uint32_t i = row + u * 64;
                 const bool in = i < s_end;
-                const uint32_t o = res_get(res, (g + it) * U + u);
+                const uint32_t f = res_get(res, (g + it) * U + u);
                 sid[it * U + u] = wa.s.rowids ? wa.s.rowids[in ? i : wa.s_lo] : i;  // wave-uniform test
-                bid[it * U + u] = build_id<IDS>(wa, rid, in && o != kNoMatch ? o : 0u);
+                bid[it * U + u] = build_id<IDS>(wa, rid, in && f != 0 ? f - 1 : 0u);
             }
         }
 #pragma unroll
@@ -646,8 +649,9 @@ __device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint16_t *
             if (walk_wave_row<U, TH>(wa, g + it) >= s_end) break;  // wave-uniform
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const uint32_t o = res_get(res, (g + it) * U + u);
-                const bool m = o != kNoMatch;
+                const uint32_t f = res_get(res, (g + it) * U + u);
+                const bool m = f != 0;
+                const uint32_t o = m ? f - 1 : 0u;
                 em.put(m, bid[it * U + u] + ob1, sid[it * U + u] + ob1, wa);
                 if (wa.dup) {  // wave-uniform
                     uint32_t x = m ? nxt[o] : 0u;
@@ -686,7 +690,7 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
     constexpr bool IDS = table_keeps_ids(KB);
     uint32_t *rid = reinterpret_cast<uint32_t *>(nxt + CAP);  // [CAP] build rows' ids (IDS only)
     __shared__ ProbeTask t;
-    uint32_t res[ProbeShape<PV>::RES];  // matched build row per probe row, two per word
+    uint32_t res[ProbeShape<PV>::RES];  // per probe row its matched build row + 1 (0: none), two per word
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     Emitter em;

@@ -544,8 +544,6 @@ __device__ __forceinline__ uint32_t probe_walk(const WalkArgs &wa, const HashCfg
         if (!PF && it > 0) walk_load<U, TH, KW, NC>(wa, it, nk);
 #pragma unroll
         for (int u = 0; u < U; u++) sk[u] = nk[u];
-#pragma unroll
-        for (int u = 0; u < U; u += 2) res[(it * U + u) >> 1] = 0;  // a row is found at most once: its field is or-ed in
         if (PF && it + 1 < ITERS) walk_load<U, TH, KW, NC>(wa, it + 1, nk);
 #pragma unroll
         for (int u = 0; u < U; u++)
@@ -630,7 +628,8 @@ __device__ __forceinline__ void replay_walk(const WalkArgs &wa, const uint16_t *
         if (walk_wave_row<U, TH>(wa, g) >= s_end) break;  // wave-uniform
         // ids of every row, matched or not, with clamped indexes: no per-row
         // branches, and the loads of a group are issued together (rows past
-        // the chunk hold stale results: they read row s_lo / build row 0)
+        // the chunk, and all rows of iterations the walk skipped, hold no
+        // match: they read row s_lo / build row 0)
         uint32_t sid[G * U], bid[G * U];
 #pragma unroll
         for (int it = 0; it < G; it++) {
@@ -836,6 +835,8 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
         // find and count every row's matches (kept in registers), reserve the
         // task's output rows with one atomic, then write them
         const WalkArgs wa = walk_args(t);
+#pragma unroll
+        for (int k = 0; k < ProbeShape<PV>::RES; k++) res[k] = 0;  // results of skipped iterations read as no match
         const uint32_t wc = probe_walk<HP, KW, NC, PV>(wa, hc, rk, slots, nxt, res);
         if (lane == 0) t.wave_count[w] = wc;
         __syncthreads();
@@ -848,9 +849,9 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
         if (wc) {
             em.pos = t.out_base;
             for (uint32_t k = 0; k < w; k++) em.pos += t.wave_count[k];
-            // row-id groups of two iterations where registers allow (single 8-byte key, FNV / XOR), of
-            // four (eight in the wide block) where the build rows' ids come from LDS (keys wider than 8 bytes)
-            replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 2 : (IDS ? (PV ? 8 : 4) : 1), PV, IDS>(wa, nxt, rid, res, em);
+            // row-id groups of four iterations for single keys of up to 8 bytes (FNV / XOR) and where the
+            // build rows' ids come from LDS (keys wider than 8 bytes; eight in the wide block)
+            replay_walk<(KB <= 8 && HP != HP_GENERIC) ? 4 : (IDS ? (PV ? 8 : 4) : 1), PV, IDS>(wa, nxt, rid, res, em);
         }
         __syncthreads();
     }

@@ -391,6 +391,12 @@ struct ProbeTask {
     uint32_t dup;
     uint32_t done;
     uint32_t wave_count[kMaxProbeWaves];
+    // the join whose fixed fields the record holds (~0: none yet), its
+    // sides' partition start / end offsets in base, chunk sizes, first task
+    // and per-partition task prefix
+    uint32_t jn, rb, re, sb, se, cap_r, cap_s;
+    unsigned long long task0;
+    const unsigned long long *pt;
 };
 
 constexpr int kProbeUnroll = 2;
@@ -700,6 +706,7 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
     // Lane 0 of wave 0 holds the ticket of the block's next task, fetched
     // while the current one runs.
     unsigned long long next_task = ~0ull;
+    if (tid == 0) t.jn = ~0u;  // read and written by this thread only (lane 0 of wave 0)
     while (true) {
         if (w == 0) {
             unsigned long long task = 0;
@@ -718,44 +725,55 @@ __global__ __launch_bounds__(probe_threads(PV), (probe_waves_per_simd<HP, KW * N
                     const JoinDesc &jd0 = a.joins[jn];
                     q = wave_upper_index(a.ptask + jd0.ptask_off, jd0.nparts, task - a.task0[jn]);
                 }
-                const JoinDesc &jd = a.joins[jn];
-                const unsigned long long *pt = a.ptask + jd.ptask_off;
-                const unsigned long long local = task - a.task0[jn];
                 if (lane == 0) {
-                    const JobDesc &jr = a.jobs[jd.job_r];
-                    const JobDesc &js = a.jobs[jd.job_s];
-                    const uint32_t r0 = a.base[jr.base_off + q], s0 = a.base[js.base_off + q];
-                    const uint32_t nr = a.base[jr.end_off + q] - r0, ns = a.base[js.end_off + q] - s0;
-                    const uint32_t s_chunks = (ns + jd.cap_s - 1) / jd.cap_s;
-                    const uint32_t k = (uint32_t)(local - pt[q]);
+                    if (jn != t.jn) {  // a join's fixed fields, kept while its tasks follow each other
+                        const JoinDesc &jd = a.joins[jn];
+                        const JobDesc &jr = a.jobs[jd.job_r];
+                        const JobDesc &js = a.jobs[jd.job_s];
+                        if (jr.b1) {
+                            for (int c = 0; c < 4; c++) t.r.cols[c] = jr.fin_cols[c];
+                            t.r.rowids = jr.fin_ids;
+                        } else {
+                            t.r = jr.in;
+                        }
+                        if (js.b1) {
+                            for (int c = 0; c < 4; c++) t.s.cols[c] = js.fin_cols[c];
+                            t.s.rowids = js.fin_ids;
+                        } else {
+                            t.s = js.in;
+                        }
+                        t.rb = jr.base_off;
+                        t.re = jr.end_off;
+                        t.sb = js.base_off;
+                        t.se = js.end_off;
+                        t.cap_r = jd.cap_r;
+                        t.cap_s = jd.cap_s;
+                        t.task0 = a.task0[jn];
+                        t.pt = a.ptask + jd.ptask_off;
+                        t.shift = jd.slot_shift;
+                        t.one_based = jd.one_based;
+                        t.out_r = jd.out_r;
+                        t.out_s = jd.out_s;
+                        t.cap = jd.cap;
+                        t.count = &a.out_count[jn];
+                        t.jn = jn;
+                    }
+                    // the partition's sides and the task's chunks: one round of loads
+                    const uint32_t cap_r = t.cap_r, cap_s = t.cap_s;
+                    const uint32_t r0 = a.base[t.rb + q], s0 = a.base[t.sb + q];
+                    const uint32_t nr = a.base[t.re + q] - r0, ns = a.base[t.se + q] - s0;
+                    const uint32_t s_chunks = (ns + cap_s - 1) / cap_s;
+                    const uint32_t k = (uint32_t)(task - t.task0 - t.pt[q]);
                     const uint32_t rc = k / s_chunks, sc = k - rc * s_chunks;
-                    if (jr.b1) {
-                        for (int c = 0; c < 4; c++) t.r.cols[c] = jr.fin_cols[c];
-                        t.r.rowids = jr.fin_ids;
-                    } else {
-                        t.r = jr.in;
-                    }
-                    if (js.b1) {
-                        for (int c = 0; c < 4; c++) t.s.cols[c] = js.fin_cols[c];
-                        t.s.rowids = js.fin_ids;
-                    } else {
-                        t.s = js.in;
-                    }
-                    t.r_lo = r0 + rc * jd.cap_r;
-                    t.r_n = nr - rc * jd.cap_r < jd.cap_r ? nr - rc * jd.cap_r : jd.cap_r;
-                    t.s_lo = s0 + sc * jd.cap_s;
-                    t.s_n = ns - sc * jd.cap_s < jd.cap_s ? ns - sc * jd.cap_s : jd.cap_s;
+                    t.r_lo = r0 + rc * cap_r;
+                    t.r_n = nr - rc * cap_r < cap_r ? nr - rc * cap_r : cap_r;
+                    t.s_lo = s0 + sc * cap_s;
+                    t.s_n = ns - sc * cap_s < cap_s ? ns - sc * cap_s : cap_s;
                     uint32_t ns2 = 64;
                     // load <= 1/4 where the slot array (2 * CAP) allows, else <= 1/2:
                     // a probe round costs the wave's longest chain, not the mean
                     while (ns2 < 4 * t.r_n && ns2 < 2 * (uint32_t)CAP) ns2 <<= 1;
                     t.nslots = ns2;
-                    t.shift = jd.slot_shift;
-                    t.one_based = jd.one_based;
-                    t.out_r = jd.out_r;
-                    t.out_s = jd.out_s;
-                    t.cap = jd.cap;
-                    t.count = &a.out_count[jn];
                     t.dup = 0;
                     t.done = 0;
                 }
